@@ -1,0 +1,178 @@
+// galp_main.cpp -- the `galp [options] method policy data [test]` program on
+// the MI355X C-ABI.
+//
+// Same command line, defaults, stdout lines and stderr TSV record as
+// label_propagation/src/main.cu:25-246, so experiments/*.sh keep working:
+//   stdout "Data is ready on the host (n, m)"      main.cu:135
+//   stdout "----------<proc>----------"             main.cu:208
+//   stdout "%d: %f (s), updated %d" per iteration   lfht.cuh:95
+//   stderr "<proc>\t<file>\t<niter>\t<b>\t<f1>\t<f2>\t<nmi>\t<fm>\t<ari>"  main.cu:240-242
+// Exit status 1 on any failure (the stop signal of experiments/common.sh:47).
+// Policies that need more than one GPU or the CPU (3, 9, 10, -g > 1) and the
+// serial CPU path (method*5+policy outside 0..10) are not part of this
+// one-GPU build: they print a message and exit 1.
+
+#include <getopt.h>
+#include <libgen.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "galp.h"
+#include "galp_host.h"
+
+static void usage(const char *prog) {
+    std::fprintf(stderr,
+                 "Usage: %s [options] method policy data [test]\n"
+                 "  method: Specify the method to be used.\n"
+                 "    0 -- data-parallel primitives (sort)\n"
+                 "    1 -- lock-free hash tables\n"
+                 "  policy: Specify the policy to perform the method.\n"
+                 "    0 -- GPU in-core\n"
+                 "    1 -- Out-of-core without overlap\n"
+                 "    2 -- Out-of-core with overlap\n"
+                 "    4 -- method=0: Load-imbalanced\n"
+                 "  data: The graph data file (.txt/.dat edge list or .bin int32 pairs).\n"
+                 "  test: The ground-truth file. Accuracy is computed if supplied.\n"
+                 "Options:\n"
+                 " -b n: Set buffer sizes to 2^n. The default is 24.\n"
+                 " -i n: Specify the number of iterations. The default is 10.\n"
+                 " -g n: Use n GPUs. The default is 1 (the only value this build runs).\n"
+                 " -l n: Hash variant 0..3 (naive, fused, LDS+spill, load-balanced). Default 1.\n"
+                 " --dispatch plain|learned  kernel choice by degree thresholds or the MLP\n"
+                 " --no-reorder              keep the input vertex numbering internally\n"
+                 " --labels FILE             write the final labels, one per line\n",
+                 prog);
+    std::exit(1);
+}
+
+int main(int argc, char **argv) {
+    int buffer_pow = 24, niter = 10, ngpus = 1, lfht_policy = 1;
+    int dispatch = GALP_DISPATCH_PLAIN, reorder = 1;
+    const char *labels_file = nullptr;
+    static const option longopts[] = {{"dispatch", required_argument, nullptr, 1000},
+                                      {"no-reorder", no_argument, nullptr, 1001},
+                                      {"labels", required_argument, nullptr, 1002},
+                                      {nullptr, 0, nullptr, 0}};
+    int ch;
+    while ((ch = getopt_long(argc, argv, "b:i:g:l:", longopts, nullptr)) != -1) {
+        switch (ch) {
+        case 'b': buffer_pow = std::atoi(optarg); break;
+        case 'i': niter = std::atoi(optarg); break;
+        case 'g': ngpus = std::atoi(optarg); break;
+        case 'l': lfht_policy = std::atoi(optarg); break;
+        case 1000:
+            if (!std::strcmp(optarg, "plain")) dispatch = GALP_DISPATCH_PLAIN;
+            else if (!std::strcmp(optarg, "learned")) dispatch = GALP_DISPATCH_LEARNED;
+            else usage(argv[0]);
+            break;
+        case 1001: reorder = 0; break;
+        case 1002: labels_file = optarg; break;
+        default: usage(argv[0]);
+        }
+    }
+    int rest = argc - optind;
+    if (rest < 3) usage(argv[0]);
+    char **args = argv + optind;
+    const int method = std::atoi(args[0]);
+    const int policy = std::atoi(args[1]);
+    const char *graph_file = args[2];
+
+    const int key = method * 5 + policy;
+    if (key < 0 || key > 10) {
+        std::printf("The serial CPU path (method*5+policy outside 0..10) is not part of the MI355X build.\n");
+        return 1;
+    }
+    if (key == 3 || key == 8) {
+        std::printf("Policy 3 (CPU-GPU hybrid) is not part of the one-GPU MI355X build.\n");
+        return 1;
+    }
+    if (key == 9 || key == 10 || ngpus != 1) {
+        std::printf("Multi-GPU runs are not part of the one-GPU MI355X build (-g must be 1).\n");
+        return 1;
+    }
+
+    galp_ctx *ctx = nullptr;
+    if (galp_open(0, &ctx) != GALP_OK) {
+        std::printf("No usable GPU (galp_open failed).\n");
+        return 1;
+    }
+
+    galp_host_csr *g = nullptr;
+    if (galp_load_graph(graph_file, &g) != GALP_OK) {
+        std::printf("Cannot load %s: %s\n", graph_file, galp_host_last_error());
+        return 1;
+    }
+    std::printf("Data is ready on the host (%lld, %lld)\n", (long long)g->n, (long long)g->m);
+
+    std::string proc_name;
+    switch (key) {
+    case 0: buffer_pow = 0; proc_name = "dpp\tincore"; break;
+    case 1: proc_name = "dpp\tsync"; break;
+    case 2: proc_name = "dpp\tasync"; break;
+    case 4: buffer_pow = 0; proc_name = "dpp\tli"; break;
+    case 5: buffer_pow = 0; proc_name = "lfht" + std::to_string(lfht_policy) + "\tincore"; break;
+    case 6: proc_name = "lfht" + std::to_string(lfht_policy) + "\tsync"; break;
+    case 7: proc_name = "lfht" + std::to_string(lfht_policy) + "\tasync"; break;
+    }
+
+    galp_opts o;
+    galp_opts_default(&o);
+    o.method = method;
+    o.policy = policy;
+    o.variant = lfht_policy;
+    o.buffer_pow = buffer_pow;
+    o.dispatch = dispatch;
+    o.reorder = reorder;
+    galp_csr_view view{g->n, g->m, g->offsets, g->neighbors, 1, 0};
+    galp_graph *dg = nullptr;
+    int rc = galp_graph_upload(ctx, &view, &o, &dg);
+    if (rc == GALP_ENOMEM) {
+        std::printf("Not enough GPU memory!!\n  %s\n", galp_last_error(ctx));
+        return 1;
+    }
+    if (rc != GALP_OK) {
+        std::printf("galp: %s\n", galp_last_error(ctx));
+        return 1;
+    }
+    std::printf("----------%s----------\n", proc_name.c_str());
+    std::fflush(stdout);
+    std::vector<int32_t> labels((size_t)g->n);
+    galp_run_result res{};
+    rc = galp_lp_run(dg, niter, 1, labels.data(), &res);
+    if (rc != GALP_OK) {
+        std::printf("galp: %s\n", galp_last_error(ctx));
+        return 1;
+    }
+
+    double nmi = 0.0, fm = 0.0, ari = 0.0;
+    if (rest == 4) {
+        if (galp_accuracy(args[3], labels.data(), (int64_t)labels.size(), &nmi, &fm, &ari) != GALP_OK) {
+            std::printf("Cannot read the ground-truth file %s\n", args[3]);
+            return 1;
+        }
+    }
+    if (labels_file) {
+        FILE *f = std::fopen(labels_file, "w");
+        if (!f) {
+            std::printf("Cannot write %s\n", labels_file);
+            return 1;
+        }
+        for (int32_t l : labels) std::fprintf(f, "%d\n", l);
+        std::fclose(f);
+    }
+
+    std::string fname(graph_file);
+    std::vector<char> buf(fname.begin(), fname.end());
+    buf.push_back('\0');
+    std::fprintf(stderr, "%s\t%s\t%d\t%d\t%f\t%f\t%f\t%f\t%f\n", proc_name.c_str(), basename(buf.data()), niter,
+                 buffer_pow, res.f1_s, res.f2_s, nmi, fm, ari);
+
+    galp_graph_free(dg);
+    galp_host_csr_free(g);
+    galp_close(ctx);
+    return 0;
+}

@@ -1,0 +1,87 @@
+// device_common.hpp -- wave64 helpers shared by the gfx950 LP kernels.
+//
+// Everything here is written for CDNA wavefronts of 64 lanes: 64-bit ballot
+// masks, lane-index shuffles and reductions across full waves or aligned
+// power-of-two lane groups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace galp {
+
+constexpr int kWave = 64;
+constexpr uint32_t kEmpty = 0xffffffffu;   // labels are vertex ids < 2^31
+
+// Fibonacci (multiplicative) hashing into a power-of-two table of 2^bits
+// slots (Knuth, TAOCP vol. 3, 6.4).  Labels are arbitrary vertex ids, and the
+// top bits of the product spread consecutive ids across the table.
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key, int bits) {
+    return (key * 0x9E3779B1u) >> (32 - bits);
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// Max over an aligned group of G lanes (G a power of two, G <= 64).
+template <int G, typename T>
+__device__ __forceinline__ T group_max(T x) {
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) {
+        T y = __shfl_xor(x, s, G);
+        x = x > y ? x : y;
+    }
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T x) { return group_max<kWave, T>(x); }
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+    for (int s = kWave / 2; s > 0; s >>= 1) x += __shfl_xor(x, s, kWave);
+    return x;
+}
+
+// Block-wide max of a uint32 / uint64 value; every thread gets the result.
+// `scratch` holds one entry per wave.  Contains __syncthreads().
+template <typename T>
+__device__ __forceinline__ T block_max(T x, T *scratch) {
+    x = wave_max(x);
+    const int nw = (int)(blockDim.x / kWave);
+    if (nw == 1) return x;
+    const int w = (int)(threadIdx.x / kWave);
+    if (lane_id() == 0) scratch[w] = x;
+    __syncthreads();
+    T r = scratch[0];
+    for (int i = 1; i < nw; ++i) r = r > scratch[i] ? r : scratch[i];
+    __syncthreads();
+    return r;
+}
+
+// Adds a per-thread count into *counter with one atomic per block.
+__device__ __forceinline__ void block_count_add(uint32_t mine, unsigned long long *counter,
+                                                uint32_t *scratch) {
+    uint32_t s = wave_sum_u32(mine);
+    const int nw = (int)(blockDim.x / kWave);
+    const int w = (int)(threadIdx.x / kWave);
+    if (nw == 1) {
+        if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
+        return;
+    }
+    __syncthreads();
+    if (lane_id() == 0) scratch[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < nw; ++i) t += scratch[i];
+        if (t) atomicAdd(counter, t);
+    }
+}
+
+// Loads that bypass the per-CU vector L1, for table words other workgroups
+// update with device-scope atomics.
+__device__ __forceinline__ uint32_t load_coherent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace galp

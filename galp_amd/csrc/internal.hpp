@@ -1,0 +1,142 @@
+// internal.hpp -- host-side state behind the galp C-ABI (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "galp.h"
+
+namespace galp {
+
+// Degree classes of the plain dispatcher.  Each class is one contiguous id
+// range after the degree-descending renumbering (or one vertex list without
+// it) and one kernel configuration.
+enum BucketId {
+    B_G4 = 0,   // 1..4      lanes-per-vertex groups of 4
+    B_G8,       // 5..8
+    B_G16,      // 9..16
+    B_G32,      // 17..32
+    B_G64,      // 33..64    one full wave per vertex
+    B_WAVE,     // 65..512   one wave per vertex, LDS table
+    B_BLOCK,    // 513..4096 one 256-thread workgroup per vertex, LDS table
+    B_HUB,      // > 4096    2048-edge tiles merged in a global table
+    NUM_BUCKETS
+};
+extern const int64_t kBucketHi[NUM_BUCKETS];
+extern const char *const kBucketName[NUM_BUCKETS];
+
+constexpr int kHubTile = 2048;        // edges per hub tile
+constexpr int kHubTileCapBits = 12;   // LDS table of a hub tile (4096 slots)
+
+struct BucketRange {
+    int64_t v0 = 0;      // first renumbered id, or first index into vlist
+    int64_t nv = 0;      // vertices in the class
+    int64_t edges = 0;   // sum of their degrees
+};
+
+struct HubTile {
+    int32_t hub;         // index into the hub arrays
+    int32_t vtx;         // vertex id (internal numbering)
+    int64_t e0, e1;      // absolute edge range [e0, e1)
+};
+
+struct KernelAgg {
+    std::string name;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+    double alg_bytes = 0.0;   // algorithmic bytes summed over launches
+};
+
+struct PendingLaunch {
+    int agg;                  // index into Profiler::aggs
+    hipEvent_t e0, e1;
+    double alg_bytes;
+};
+
+struct Profiler {
+    bool on = false;
+    std::vector<KernelAgg> aggs;
+    std::vector<PendingLaunch> pending;
+    std::vector<hipEvent_t> pool;
+    int find(const char *name);
+};
+
+}  // namespace galp
+
+struct galp_ctx {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    std::string err;
+};
+
+struct galp_graph {
+    galp_ctx *ctx = nullptr;
+    galp_opts opts{};
+    int64_t n = 0, m = 0;
+    bool off64 = false;
+    bool reorder = false;
+    hipStream_t stream = nullptr;
+
+    // CSR in HBM (internal numbering when reorder is on)
+    void *d_off = nullptr;          // int32 or int64, n+1
+    int32_t *d_nbr = nullptr;       // m
+    int32_t *d_lab[2] = {nullptr, nullptr};
+    int cur = 0;                    // d_lab[cur] holds the current labels
+    int32_t *d_perm = nullptr;      // original id -> internal id
+    int32_t *d_iperm = nullptr;     // internal id -> original id
+    int32_t *d_vlist = nullptr;     // vertex lists per class (no reorder)
+    int32_t *d_out = nullptr;       // staging for get_labels (n)
+
+    galp::BucketRange bucket[galp::NUM_BUCKETS];
+
+    // hub path
+    int64_t nhubs = 0, ntiles = 0, gcap = 0;
+    galp::HubTile *d_tiles = nullptr;
+    int64_t *d_hub_toff = nullptr;  // table offset per hub
+    int32_t *d_hub_tbits = nullptr; // log2 table capacity per hub
+    int32_t *d_hub_vid = nullptr;   // vertex id per hub
+    uint32_t *d_gkey = nullptr, *d_gcnt = nullptr, *d_glast = nullptr;
+
+    // iteration control (device)
+    unsigned long long *d_counter = nullptr;
+    int32_t *d_stop = nullptr;
+    int32_t *d_iters_done = nullptr;
+    long long *d_iter_counts = nullptr;  // kMaxIters
+    int32_t *d_err = nullptr;            // set by a kernel that hit an impossible state
+    int32_t *h_pinned = nullptr;         // pinned staging for small readbacks
+    int iter = 0;                        // iterations executed since init
+    bool initialized = false;
+    double preprocess_ms = 0.0;
+    int64_t device_bytes = 0;
+
+    galp::Profiler prof;
+};
+
+namespace galp {
+
+constexpr int kMaxIters = 4096;
+
+int fail(galp_ctx *ctx, int code, const std::string &msg);
+
+// Kernel launches for one LP iteration of the hash method (lp_hash.hip).
+int launch_hash_iteration(galp_graph *g);
+// Iteration bookkeeping: records the change count, sets the stop flag.
+int launch_iter_end(galp_graph *g);
+
+// Graph bookkeeping kernels (graph_dev.hip), all on g->stream.
+int dev_init_labels(galp_graph *g);
+int dev_clear_stop(galp_graph *g);
+int dev_fill_u32(galp_graph *g, uint32_t *p, int64_t n, uint32_t val);
+int dev_permute_rows(galp_graph *g, const int64_t *d_off_old, const int32_t *d_nbr_old,
+                     int32_t *d_nbr_new);
+int dev_map_labels(galp_graph *g, const int32_t *lab, int32_t *out);
+int dev_unmap_labels(galp_graph *g, const int32_t *in, int32_t *lab);
+
+// Profiling helpers (api.hip): bracket a launch with events when enabled.
+void prof_begin(galp_graph *g, const char *name, double alg_bytes, PendingLaunch *pl);
+void prof_end(galp_graph *g, PendingLaunch *pl);
+
+}  // namespace galp

@@ -1,0 +1,191 @@
+/*
+ * galp.h -- C-ABI of the MI355X (gfx950) label-propagation update.
+ *
+ * The drop-in boundary for galp's hot path: methods 0 (sort / segmented
+ * reduce) and 1 (hash tables) of the `galp [opts] method policy data [test]`
+ * program, on one GPU.  Plain C types only; every entry point returns 0 or a
+ * negative galp_status (galp_status.h) and never throws.  One galp_ctx per
+ * host thread; calls on one ctx (and its graphs) must not overlap.
+ *
+ * Semantics shared by every method, policy and variant (each a deliberate,
+ * documented difference from the reference's GPU kernels, which are
+ * asynchronous and resolve ties by atomic order):
+ *   - synchronous update: iteration t reads labels(t-1) and writes labels(t);
+ *   - tie rule: new(v) = argmax_L (count_v(L), last_v(L)), last_v(L) being the
+ *     largest CSR position of v's row holding label L -- the result of the
+ *     `if (max_count <= c)` scan of label_propagation/src/common/
+ *     hybrid_cpu_worker.cuh:177-186;
+ *   - a vertex without neighbours keeps its label;
+ *   - updated = #{v : labels(t)[v] != labels(t-1)[v]}; a run stops after an
+ *     iteration with updated <= n * 1e-5 (lfht.cuh:97, dpp.cuh:88).
+ */
+#ifndef GALP_H
+#define GALP_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include "galp_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef GALP_API
+#define GALP_API __attribute__((visibility("default")))
+#endif
+
+typedef struct galp_ctx galp_ctx;
+typedef struct galp_graph galp_graph;
+
+/* Borrowed host CSR: n+1 offsets (int64 if offsets_are_64, else int32) and m
+ * int32 neighbour ids.  The caller keeps the arrays alive for the duration of
+ * galp_graph_upload (in-core) or of the graph (out-of-core policies, which
+ * stream from them).  Mirrors CSRGraph<V,E>::{offsets, neighbors}
+ * (label_propagation/include/graph.h:160-161). */
+typedef struct galp_csr_view {
+    int64_t n;
+    int64_t m;
+    const void *offsets;
+    const int32_t *neighbors;
+    int32_t offsets_are_64;
+    int32_t reserved;
+} galp_csr_view;
+
+enum { GALP_METHOD_SORT = 0, GALP_METHOD_HASH = 1 };
+enum {
+    GALP_POLICY_INCORE = 0,     /* main.cu:140-145,169-174 */
+    GALP_POLICY_OOC_SYNC = 1,   /* out-of-core, no overlap: main.cu:147-150,176-179 */
+    GALP_POLICY_OOC_ASYNC = 2,  /* out-of-core, overlapped: main.cu:152-155,181-184 */
+    GALP_POLICY_LI = 4          /* method 0 only, "load-imbalanced": main.cu:162-167 */
+};
+enum { GALP_DISPATCH_PLAIN = 0, GALP_DISPATCH_LEARNED = 1 };
+
+/* Options of one graph.  Mirrors the CLI state of main.cu:80-116 (method,
+ * policy, -b, -l) plus the build's own switches. */
+typedef struct galp_opts {
+    int32_t method;      /* GALP_METHOD_* */
+    int32_t policy;      /* GALP_POLICY_* */
+    int32_t variant;     /* -l 0..3 (main.cu:98-100): 0 naive, 1 fused (default),
+                            2 small LDS tables + spill, 3 load-balanced tiles */
+    int32_t buffer_pow;  /* -b: out-of-core chunk of 2^buffer_pow edges (main.cu:86-88) */
+    int32_t dispatch;    /* GALP_DISPATCH_*: degree thresholds, or the MLP */
+    int32_t reorder;     /* 1 (default): degree-descending internal renumbering;
+                            labels are mapped back on every read-out */
+    int32_t profile;     /* 1: time every kernel launch with HIP events */
+    int32_t reserved;
+    int64_t mem_budget;  /* device bytes the graph may allocate; 0 = free memory.
+                            Exceeding it fails with GALP_ENOMEM (check_gpu_mem,
+                            main.cu:51-64) */
+} galp_opts;
+
+/* Per-iteration result.  Replaces the count returned by iterate()/get_count()
+ * (incore.cuh:42-48, lfht.cuh:171-179) and the printed time (lfht.cuh:95). */
+typedef struct galp_step_stats {
+    int64_t updated;     /* vertices whose label changed */
+    int64_t overflowed;  /* vertices re-run on the spill path (learned dispatch) */
+    double device_ms;    /* HIP-event time of the iteration on the graph's stream */
+    int32_t iteration;   /* 1-based index of the iteration just run */
+    int32_t stopped;     /* 1 if the n*1e-5 rule says to stop after it */
+} galp_step_stats;
+
+/* Result of a whole run.  f1/f2 keep the meaning of LabelPropagator::run's
+ * pair (lfht.cuh:76-111): f1 = main loop including the final label copy,
+ * f2 = f1 plus the preprocessing done at upload. */
+typedef struct galp_run_result {
+    double f1_s;
+    double f2_s;
+    double device_ms;    /* HIP-event time of the iterations alone */
+    int32_t iterations;  /* iterations executed (<= niter) */
+    int32_t reserved;
+} galp_run_result;
+
+typedef struct galp_graph_info {
+    int64_t n;
+    int64_t m;
+    int32_t offsets_are_64;
+    int32_t nbuckets;
+    int64_t bucket_vertices[16];
+    int64_t bucket_edges[16];
+    int64_t device_bytes;
+    double preprocess_ms;
+} galp_graph_info;
+
+typedef struct galp_device_info {
+    char name[64];
+    char arch[32];
+    int32_t compute_units;
+    int32_t wavefront_size;
+    int64_t lds_per_block;
+    int64_t total_mem;
+    int64_t l2_bytes;
+} galp_device_info;
+
+/* Per-kernel timing when galp_opts.profile = 1. */
+typedef struct galp_kernel_stat {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+    double alg_bytes;    /* algorithmic HBM bytes summed over the launches */
+} galp_kernel_stat;
+
+/* Fills the reference's defaults: method 1, policy 0, -l 1, -b 24, plain
+ * dispatch, reorder on (main.cu:80-83). */
+GALP_API void galp_opts_default(galp_opts *o);
+
+/* Opens a device.  Replaces the warm-up cudaSetDevice/cudaFree(0) of
+ * main.cu:123-128.  device must be 0 in this one-GPU build's tests, but any
+ * visible device is accepted. */
+GALP_API int galp_open(int device, galp_ctx **out);
+GALP_API int galp_close(galp_ctx *ctx);
+GALP_API int galp_device_query(galp_ctx *ctx, galp_device_info *out);
+
+/* Copies the CSR to HBM and prepares the dispatch plan.  Replaces
+ * init_gmem + transfer_data (lfht.cuh:182-214, dpp.cuh:154-175,
+ * incore.cuh:34-38,59-73) and the propagator constructors selected by
+ * main.cu:139-207.  Policies 3 (hybrid), 4 with method 1 and 5 (multi-GPU)
+ * return GALP_EUNSUPPORTED, like the one-GPU build's CLI. */
+GALP_API int galp_graph_upload(galp_ctx *ctx, const galp_csr_view *csr,
+                               const galp_opts *opts, galp_graph **out);
+GALP_API int galp_graph_free(galp_graph *g);
+GALP_API int galp_graph_query(galp_graph *g, galp_graph_info *out);
+
+/* labels[i] = i.  Replaces initialize_labels (dpp/kernel.cuh:15-22). */
+GALP_API int galp_lp_init(galp_graph *g);
+
+/* Sets the current labels from host_labels[n] (original vertex numbering,
+ * every value in [0, n)) and marks the graph initialised, so the next step
+ * reads them.  No reference counterpart (the reference always starts from
+ * labels[i] = i); it lets the tests run an iteration from any label state. */
+GALP_API int galp_lp_set_labels(galp_graph *g, const int32_t *host_labels);
+
+/* One synchronous iteration.  Replaces perform_lp + get_count
+ * (lfht.cuh:115-179, dpp.cuh:104-150). */
+GALP_API int galp_lp_step(galp_graph *g, galp_step_stats *st);
+
+/* init + up to niter iterations with the n*1e-5 stop + final copy to
+ * labels_out (n entries, may be NULL).  Replaces LabelPropagator::run
+ * (lfht.cuh:73-112, dpp.cuh:64-102).  verbose = 1 prints the reference's
+ * per-iteration line "%d: %f (s), updated %d" (lfht.cuh:95) to stdout, which
+ * needs a host synchronisation per iteration; verbose = 0 queues all
+ * iterations and applies the stop rule on the device. */
+GALP_API int galp_lp_run(galp_graph *g, int32_t niter, int32_t verbose,
+                         int32_t *labels_out, galp_run_result *res);
+
+/* Copies the current labels (original vertex numbering) to host_out[n].
+ * Replaces LabelPropagator::get_labels (label_propagator.h:22-28). */
+GALP_API int galp_lp_get_labels(galp_graph *g, int32_t *host_out);
+
+/* Per-kernel timing (profile = 1).  Writes up to max entries; *count gets the
+ * number of kernels seen.  galp_profile_reset clears the totals. */
+GALP_API int galp_profile_read(galp_graph *g, galp_kernel_stat *out,
+                               int32_t max, int32_t *count);
+GALP_API int galp_profile_reset(galp_graph *g);
+
+/* Message of the last failure on ctx; "" if none. */
+GALP_API const char *galp_last_error(galp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALP_H */

@@ -1,0 +1,205 @@
+"""Parity of the HIP LP path against the CPU oracle, through the C-ABI.
+
+Bit-exact labels and change counts after every iteration, for every method,
+variant and option that changes which kernels run, on hand goldens, on
+seeded synthetic graphs covering every degree class, and on the full-size
+RMAT-22 benchmark graph.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from galp_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [_abi.METHOD_HASH]
+REORDER = [1, 0]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def check_per_iteration(ctx, off, nbr, niter, tag, **opts):
+    """Steps the GPU path and the oracle side by side from labels[i] = i."""
+    say(f"[parity] {tag} n={len(off) - 1} m={len(nbr)} opts={opts}")
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    with _abi.Graph(ctx, off, nbr, **opts) as g:
+        g.init()
+        for it in range(niter):
+            ref, cnt = oracle.lp_iterate(off, nbr, prev)
+            st = g.step()
+            got = g.labels()
+            if not np.array_equal(got, ref):
+                bad = np.nonzero(got != ref)[0]
+                raise AssertionError(f"{tag} iteration {it + 1}: {len(bad)} labels differ, first at vertex "
+                                     f"{bad[0]}: gpu {got[bad[0]]} oracle {ref[bad[0]]}")
+            assert st["updated"] == cnt, (tag, it + 1, st["updated"], cnt)
+            assert st["stopped"] == (cnt <= (len(off) - 1) * 1e-5)
+            prev = ref
+            if st["stopped"]:
+                break
+    return prev
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("reorder", REORDER)
+def test_sample_golden(ctx, method, reorder):
+    g = load("sample_labels.json")
+    off = np.array(g["offsets"], np.int64)
+    nbr = np.array(g["neighbors"], np.int32)
+    with _abi.Graph(ctx, off, nbr, method=method, reorder=reorder) as dg:
+        dg.init()
+        for it in g["iterations"]:
+            st = dg.step()
+            assert dg.labels().tolist() == it["labels"]
+            assert st["updated"] == it["updated"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("reorder", REORDER)
+def test_micro_single_steps(ctx, method, reorder):
+    for case in load("micro_cases.json")["single_step"]:
+        off = np.array(case["offsets"], np.int64)
+        nbr = np.array(case["neighbors"], np.int32)
+        with _abi.Graph(ctx, off, nbr, method=method, reorder=reorder) as dg:
+            dg.set_labels(np.array(case["prev"], np.int32))
+            st = dg.step()
+            assert dg.labels().tolist() == case["next"], case["name"]
+            assert st["updated"] == case["updated"], case["name"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("reorder", REORDER)
+def test_micro_runs_device_loop(ctx, method, reorder):
+    for case in load("micro_cases.json")["runs"]:
+        off = np.array(case["offsets"], np.int64)
+        nbr = np.array(case["neighbors"], np.int32)
+        k = case.get("stops_after", len(case["iterations"]))
+        niter = 10 if "stops_after" in case else k
+        with _abi.Graph(ctx, off, nbr, method=method, reorder=reorder) as dg:
+            labels, res = dg.run(niter)
+            assert res["iterations"] == k, case["name"]
+            assert labels.tolist() == case["iterations"][k - 1]["labels"], case["name"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("reorder", REORDER)
+def test_planted_partition_16k(ctx, method, reorder):
+    off, nbr = _abi.gen_planted_partition(16384, 256, 8, 2, 1)
+    check_per_iteration(ctx, off, nbr, 10, "pp16k", method=method, reorder=reorder)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_rmat16(ctx, method):
+    off, nbr = _abi.gen_rmat(16, 16, seed=16)
+    check_per_iteration(ctx, off, nbr, 10, "rmat16", method=method)
+
+
+def hub_graph(seed=0):
+    """Hubs of degree 5k..40k (global-table tiles), every smaller class, and
+    repeated labels so counts tie and differ."""
+    rng = np.random.default_rng(seed)
+    n = 60000
+    rows = [[] for _ in range(n)]
+    for h, d in ((0, 40000), (1, 9000), (2, 5000)):
+        for u in rng.choice(np.arange(3, n), size=d, replace=False):
+            rows[h].append(int(u))
+            rows[int(u)].append(h)
+    for v in range(3, n):
+        k = int(rng.integers(0, 12))
+        for u in rng.integers(3, n, size=k):
+            rows[v].append(int(u))
+            rows[int(u)].append(v)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(r) for r in rows])
+    nbr = np.array([u for r in rows for u in r], np.int32)
+    return off, nbr
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("reorder", REORDER)
+def test_hubs_and_duplicates(ctx, method, reorder):
+    off, nbr = hub_graph()
+    deg = np.diff(off)
+    assert deg.max() > 4096 and (deg > 512).sum() >= 3
+    check_per_iteration(ctx, off, nbr, 6, "hubs", method=method, reorder=reorder)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_hub_from_clustered_labels(ctx, method):
+    # A hub whose neighbours carry few distinct labels with many ties.
+    off, nbr = hub_graph(1)
+    n = len(off) - 1
+    rng = np.random.default_rng(4)
+    prev = rng.integers(0, 7, size=n).astype(np.int32)
+    ref, cnt = oracle.lp_iterate(off, nbr, prev)
+    with _abi.Graph(ctx, off, nbr, method=method) as g:
+        g.set_labels(prev)
+        st = g.step()
+        assert np.array_equal(g.labels(), ref)
+        assert st["updated"] == cnt
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_device_loop_equals_steps(ctx, method):
+    off, nbr = _abi.gen_rmat(14, 16, seed=3)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    with _abi.Graph(ctx, off, nbr, method=method) as g:
+        labels, res = g.run(10)
+    assert res["iterations"] == len(per)
+    assert np.array_equal(labels, final)
+
+
+def test_empty_and_edgeless(ctx):
+    for off, nbr in ((np.zeros(1, np.int64), np.zeros(0, np.int32)),
+                     (np.zeros(6, np.int64), np.zeros(0, np.int32))):
+        with _abi.Graph(ctx, off, nbr) as g:
+            labels, res = g.run(10)
+            assert labels.tolist() == list(range(len(off) - 1))
+            assert res["iterations"] == 1   # 0 changes <= n*1e-5 stops at once
+
+
+def test_errors(ctx):
+    off = np.array([0, 1, 2], np.int64)
+    with pytest.raises(_abi.GalpError) as e:
+        _abi.Graph(ctx, off, np.array([1, 5], np.int32))
+    assert e.value.code == -1
+    with pytest.raises(_abi.GalpError) as e:
+        _abi.Graph(ctx, np.array([0, 2, 1], np.int64), np.array([1, 0], np.int32))
+    assert e.value.code == -1
+    ok_nbr = np.array([1, 0], np.int32)
+    for policy, method in ((3, 1), (4, 1), (5, 1), (3, 0)):
+        with pytest.raises(_abi.GalpError) as e:
+            _abi.Graph(ctx, off, ok_nbr, policy=policy, method=method)
+        assert e.value.code == -4
+    with pytest.raises(_abi.GalpError) as e:
+        _abi.Graph(ctx, off, ok_nbr, mem_budget=64)
+    assert e.value.code == -2
+    with _abi.Graph(ctx, off, ok_nbr) as g:
+        with pytest.raises(_abi.GalpError) as e:
+            g.step()
+        assert e.value.code == -6
+        with pytest.raises(_abi.GalpError) as e:
+            g.set_labels(np.array([0, 2], np.int32))
+        assert e.value.code == -1
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_rmat22_full_size(ctx, method):
+    """The benchmark graph itself (RMAT scale 22, m = 128.3M): every iteration
+    bit-exact against the oracle."""
+    off, nbr = _abi.gen_rmat(22, 16, seed=22)
+    check_per_iteration(ctx, off, nbr, 10, "rmat22", method=method)
