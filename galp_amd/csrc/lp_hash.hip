@@ -109,6 +109,10 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
         const E b = off[v], e = off[v + 1];
         const int deg = (int)(e - b);
         const int bits = max(6, ceil_log2_dev((uint32_t)(2 * deg)));
+        if (bits > CAPBITS || deg > T * EPT) {   // outside this class: never planned here
+            if (threadIdx.x == 0) atomicOr(err, 8);
+            continue;
+        }
         const uint32_t mask = (1u << bits) - 1u;
         for (int s = threadIdx.x; s <= (int)mask; s += T) {
             skey[s] = kEmpty;
@@ -188,6 +192,10 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr,
         const int64_t rowb = (int64_t)off[tl.vtx];
         const int len = (int)(tl.e1 - tl.e0);
         const int bits = max(6, ceil_log2_dev((uint32_t)(2 * len)));
+        if (bits > kHubTileCapBits || len > kHubTile) {
+            if (threadIdx.x == 0) atomicOr(err, 16);
+            continue;
+        }
         const uint32_t mask = (1u << bits) - 1u;
         for (int s = threadIdx.x; s <= (int)mask; s += T) {
             skey[s] = kEmpty;
@@ -385,7 +393,7 @@ int launch_hash_iteration(galp_graph *g) {
 
 int launch_iter_end(galp_graph *g) {
     k_iter_end<<<1, 1, 0, g->stream>>>(g->d_counter, g->d_stop, g->d_iter_counts, g->d_iters_done,
-                                       g->iter, (double)g->n * 1e-5);
+                                       g->iter % kMaxIters, (double)g->n * 1e-5);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("iter_end launch: ") + hipGetErrorString(e));
     return GALP_OK;
