@@ -126,7 +126,7 @@ static int dev_alloc(galp_graph *g, T **p, int64_t count) {
 static void dev_free_all(galp_graph *g) {
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_tiles, g->d_hub_toff, g->d_hub_tbits, g->d_hub_vid, g->d_gkey,
-                    g->d_gcnt, g->d_glast, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts,
+                    g->d_gcnt, g->d_glast, g->d_hub_nocc, g->d_occ, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts,
                     g->d_err};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -287,7 +287,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     // ---- memory check (check_gpu_mem, main.cu:51-64) --------------------
     {
         const int64_t esz = g->off64 ? 8 : 4;
-        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 12 * gcap + (int64_t)sizeof(HubTile) * g->ntiles +
+        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 16 * gcap + (int64_t)sizeof(HubTile) * g->ntiles +
                        16 * g->nhubs + 4096;
         if (g->reorder) need += 8 * n + 4 * m + 8 * (n + 1);  // perm arrays + transient original CSR
         else need += 4 * n;
@@ -355,6 +355,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_gkey, gcap));
         GALP_TRY(dev_alloc(g, &g->d_gcnt, gcap));
         GALP_TRY(dev_alloc(g, &g->d_glast, gcap));
+        GALP_TRY(dev_alloc(g, &g->d_occ, gcap));
+        GALP_TRY(dev_alloc(g, &g->d_hub_nocc, g->nhubs));
         GALP_HIP(ctx, hipMemcpy(g->d_tiles, tiles.data(), sizeof(HubTile) * tiles.size(), hipMemcpyHostToDevice));
         GALP_HIP(ctx, hipMemcpy(g->d_hub_toff, hub_toff.data(), 8 * hub_toff.size(), hipMemcpyHostToDevice));
         GALP_HIP(ctx, hipMemcpy(g->d_hub_tbits, hub_tbits.data(), 4 * hub_tbits.size(), hipMemcpyHostToDevice));
@@ -362,6 +364,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_fill_u32(g, g->d_gkey, gcap, 0xffffffffu));
         GALP_TRY(dev_fill_u32(g, g->d_gcnt, gcap, 0u));
         GALP_TRY(dev_fill_u32(g, g->d_glast, gcap, 0u));
+        GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_nocc, g->nhubs, 0u));
     }
     GALP_HIP(ctx, hipStreamSynchronize(g->stream));
     g->preprocess_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -42,39 +42,45 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
     return x;
 }
 
-// Block-wide max of a uint32 / uint64 value; every thread gets the result.
-// `scratch` holds one entry per wave.  Contains __syncthreads().
-template <typename T>
+// Block-wide max over NW waves; every thread gets the result.  The wave
+// count is a template argument: blockDim is read from the dispatch packet
+// with a vector load, which would drain every load in flight.
+// `scratch` holds NW entries.  Contains barriers when NW > 1.
+template <int NW, typename T>
 __device__ __forceinline__ T block_max(T x, T *scratch) {
     x = wave_max(x);
-    const int nw = (int)(blockDim.x / kWave);
-    if (nw == 1) return x;
-    const int w = (int)(threadIdx.x / kWave);
-    if (lane_id() == 0) scratch[w] = x;
-    __syncthreads();
-    T r = scratch[0];
-    for (int i = 1; i < nw; ++i) r = r > scratch[i] ? r : scratch[i];
-    __syncthreads();
-    return r;
+    if constexpr (NW == 1) {
+        return x;
+    } else {
+        const int w = (int)(threadIdx.x / kWave);
+        if (lane_id() == 0) scratch[w] = x;
+        __syncthreads();
+        T r = scratch[0];
+#pragma unroll
+        for (int i = 1; i < NW; ++i) r = r > scratch[i] ? r : scratch[i];
+        __syncthreads();
+        return r;
+    }
 }
 
-// Adds a per-thread count into *counter with one atomic per block.
+// Adds a per-thread count into *counter with one atomic per block of NW waves.
+template <int NW>
 __device__ __forceinline__ void block_count_add(uint32_t mine, unsigned long long *counter,
                                                 uint32_t *scratch) {
     uint32_t s = wave_sum_u32(mine);
-    const int nw = (int)(blockDim.x / kWave);
-    const int w = (int)(threadIdx.x / kWave);
-    if (nw == 1) {
+    if constexpr (NW == 1) {
         if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
-        return;
-    }
-    __syncthreads();
-    if (lane_id() == 0) scratch[w] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < nw; ++i) t += scratch[i];
-        if (t) atomicAdd(counter, t);
+    } else {
+        const int w = (int)(threadIdx.x / kWave);
+        __syncthreads();
+        if (lane_id() == 0) scratch[w] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = 0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) t += scratch[i];
+            if (t) atomicAdd(counter, t);
+        }
     }
 }
 

@@ -15,14 +15,18 @@ namespace galp {
 // range after the degree-descending renumbering (or one vertex list without
 // it) and one kernel configuration.
 enum BucketId {
-    B_G4 = 0,   // 1..4      lanes-per-vertex groups of 4
+    B_G4 = 0,   // 1..4       lanes-per-vertex groups of 4
     B_G8,       // 5..8
     B_G16,      // 9..16
     B_G32,      // 17..32
-    B_G64,      // 33..64    one full wave per vertex
-    B_WAVE,     // 65..512   one wave per vertex, LDS table
-    B_BLOCK,    // 513..4096 one 256-thread workgroup per vertex, LDS table
-    B_HUB,      // > 4096    2048-edge tiles merged in a global table
+    B_G64,      // 33..64     one full wave per vertex
+    B_W128,     // 65..128    one wave per vertex, 256-slot LDS table
+    B_W256,     // 129..256   one wave, 512 slots
+    B_W512,     // 257..512   one wave, 1024 slots
+    B_B1K,      // 513..1024  128 threads, 2048 slots
+    B_B2K,      // 1025..2048 256 threads, 4096 slots
+    B_B4K,      // 2049..4096 512 threads, 8192 slots
+    B_HUB,      // > 4096     2048-edge tiles merged in a global table
     NUM_BUCKETS
 };
 extern const int64_t kBucketHi[NUM_BUCKETS];
@@ -99,6 +103,8 @@ struct galp_graph {
     int32_t *d_hub_tbits = nullptr; // log2 table capacity per hub
     int32_t *d_hub_vid = nullptr;   // vertex id per hub
     uint32_t *d_gkey = nullptr, *d_gcnt = nullptr, *d_glast = nullptr;
+    int32_t *d_hub_nocc = nullptr;  // occupied slots per hub table
+    int32_t *d_occ = nullptr;       // occupied slot list, same layout as the tables
 
     // iteration control (device)
     unsigned long long *d_counter = nullptr;

@@ -9,35 +9,89 @@
 // two memsets every iteration.  Here the vertices are split into degree
 // classes, each with its own kernel:
 //   - degree <= 64: aligned groups of G = 4..64 lanes per vertex; counts and
-//     last positions come from G lane shuffles, no table at all;
-//   - 65..4096: one wave (or one 256-thread workgroup) per vertex with an
-//     open-addressing table in LDS sized to the vertex's degree;
+//     last positions come from lane exchanges, no table at all;
+//   - 65..4096: one wave (or one workgroup of up to 512 threads) per vertex
+//     with an open-addressing table in LDS at load factor <= 1/2;
 //   - hubs (> 4096): 2048-edge tiles aggregate in LDS, then merge (count add,
-//     last max) into a per-hub global table; a finalize kernel takes the
-//     argmax and clears the table for the next iteration (no memsets).
+//     last max) into a per-hub global table whose occupied slots are listed;
+//     a finalize kernel reduces only the listed slots and clears them.
+// Table-based kernels run a software pipeline over their vertices: the
+// labels of the next vertex and the neighbour ids of the one after are in
+// flight while the current vertex is counted, so the dependent
+// offsets -> neighbours -> labels chain is hidden.  Tables are cleared
+// lazily (each lane resets the slots it used), never with a memset.
 // All kernels are synchronous (read prev, write next) and apply the tie rule
-// argmax (count, last position), so the result is independent of scheduling
-// and of atomic order.
+// argmax (count, last position), independent of scheduling and atomic order.
 
 #include "device_common.hpp"
 #include "internal.hpp"
 
 namespace galp {
 
-const int64_t kBucketHi[NUM_BUCKETS] = {4, 8, 16, 32, 64, 512, 4096, INT64_MAX};
-const char *const kBucketName[NUM_BUCKETS] = {"g4", "g8", "g16", "g32", "g64", "wave", "block", "hub"};
-
-__device__ __forceinline__ int ceil_log2_dev(uint32_t x) {
-    return x <= 1 ? 0 : 32 - __clz((int)(x - 1));
-}
+const int64_t kBucketHi[NUM_BUCKETS] = {4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, INT64_MAX};
+const char *const kBucketName[NUM_BUCKETS] = {"g4",   "g8",   "g16",  "g32",  "g64",  "w128",
+                                              "w256", "w512", "b1k",  "b2k",  "b4k",  "hub"};
 
 // ---------------------------------------------------------------------------
-// Degree <= G: one aligned group of G lanes per vertex.  Lane p holds the
-// label of the p-th neighbour; a sweep of G shuffles gives, per lane, the
-// count of its label and the last position holding it.  The group max of
-// (count << 8 | last) is the tie rule; the winning label is read back from
-// lane `last`.
-template <int G, typename E>
+// Degree <= G: one aligned group of G lanes per vertex; lane p holds the
+// label of the p-th neighbour (or -1).  The group's max of (count << 8 |
+// last) is found in two stages:
+//   peel:  up to PEEL rounds; in each round every group takes the label of
+//          its first unresolved lane and one ballot gives that label's count
+//          (popcount) and last position (highest set bit), so the labels
+//          that dominate a neighbourhood are settled in one round each;
+//   sweep: lanes still unresolved (a group with more than PEEL distinct
+//          labels) compare against the G lanes of their group.
+// A peeled label takes all of its lanes with it, so the sweep's counts
+// among unresolved lanes are full counts.
+template <int G, int PEEL>
+__device__ __forceinline__ uint32_t group_best(int32_t L) {
+    const int lane = lane_id();
+    const int gbase = lane & ~(G - 1);
+    const unsigned long long gbits = (G == kWave) ? ~0ull : (((1ull << G) - 1ull) << gbase);
+    bool active = L >= 0;
+    uint32_t best = 0;
+#pragma unroll
+    for (int r = 0; r < PEEL; ++r) {
+        const unsigned long long ball = __ballot(active);
+        if (ball == 0) break;
+        int32_t Lk;
+        if constexpr (G == kWave) {
+            Lk = __builtin_amdgcn_readlane(L, __ffsll((long long)ball) - 1);
+        } else {
+            const unsigned long long seg = ball & gbits;
+            Lk = __shfl(L, seg ? __ffsll((long long)seg) - 1 : lane);
+        }
+        const bool eq = active && (L == Lk);
+        const unsigned long long m = __ballot(eq) & gbits;
+        if (m) {
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            const uint32_t last = (uint32_t)(63 - __clzll((long long)m) - gbase);
+            best = max(best, (cnt << 8) | last);
+        }
+        active = active && !eq;
+    }
+    if (__ballot(active)) {
+        const int32_t La = active ? L : -1;
+        uint32_t cnt = 0, last = 0;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            int32_t Lj;
+            if constexpr (G == kWave) Lj = __builtin_amdgcn_readlane(La, j);
+            else Lj = __shfl(La, j, G);
+            const bool eq = active && (Lj == L);
+            cnt += eq;
+            last = eq ? (uint32_t)j : last;
+        }
+        best = max(best, group_max<G>(active ? ((cnt << 8) | last) : 0u));
+    }
+    return best;
+}
+
+// One wave holds 64/G vertices per round; rounds are software-pipelined:
+// while round t is resolved, the labels of round t+1 and the neighbour ids
+// of round t+2 are loading, and the offsets of round t+3.
+template <int G, int PEEL, bool VLIST, typename E>
 __global__ void __launch_bounds__(256)
 k_hash_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
              const int32_t *__restrict__ prev, int32_t *__restrict__ next,
@@ -46,50 +100,287 @@ k_hash_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
     __shared__ uint32_t scratch[256 / kWave];
     if (*stop) return;
     const int p = lane_id() & (G - 1);
-    const int64_t per_block = blockDim.x / G;
+    constexpr int64_t per_block = 256 / G;
+    const int64_t stride = (int64_t)gridDim.x * per_block;
+    // Raw offsets stay in the pipeline registers until the next stage uses
+    // them (any arithmetic on a just-loaded value would wait for it), and
+    // every load is issued by every lane from a clamped address, so no load
+    // sits under a branch (see pipelined()).
+    struct St {
+        int64_t v;
+        E ob, oe;     // off[v], off[v+1], raw
+        bool ok;      // i < nv
+    };
+    auto state = [&](int64_t i) {
+        St s;
+        s.ok = i < nv;
+        const int64_t ic = s.ok ? i : 0;
+        if constexpr (VLIST) s.v = (int64_t)vlist[v0 + ic];
+        else s.v = v0 + ic;
+        s.ob = off[s.v];
+        s.oe = off[s.v + 1];
+        return s;
+    };
+    auto has = [&](const St &s) { return s.ok && (int64_t)p < (int64_t)(s.oe - s.ob); };
+    auto ldn = [&](const St &s) -> int32_t {
+        return __builtin_nontemporal_load(&nbr[has(s) ? (int64_t)s.ob + p : 0]);
+    };
+    auto gat = [&](int32_t u, const St &s) -> int32_t { return prev[has(s) ? u : 0]; };
     uint32_t changed = 0;
-    for (int64_t base = (int64_t)blockIdx.x * per_block; base < nv;
-         base += (int64_t)gridDim.x * per_block) {
-        const int64_t i = base + threadIdx.x / G;
-        const bool vok = i < nv;
-        int64_t v = 0;
-        E b = 0, e = 0;
-        if (vok) {
-            v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
-            b = off[v];
-            e = off[v + 1];
-        }
-        const int deg = (int)(e - b);
-        const bool ok = p < deg;
-        const int32_t L = ok ? prev[nbr[b + p]] : -1;
-        uint32_t cnt = 0;
-        int last = 0;
-#pragma unroll
-        for (int j = 0; j < G; ++j) {
-            const int32_t Lj = __shfl(L, j, G);
-            const bool eq = (Lj == L) & (j < deg);
-            cnt += eq;
-            last = eq ? j : last;
-        }
-        uint32_t key = ok ? ((cnt << 8) | (uint32_t)last) : 0u;
-        key = group_max<G>(key);
-        const int32_t win = __shfl(L, (int)(key & 0xffu), G);
-        if (ok && p == 0) {
-            const int32_t old = prev[v];
-            next[v] = win;
+    auto resolve = [&](const St &s, int32_t Lraw, int32_t old) {
+        const int32_t L = has(s) ? Lraw : -1;
+        const uint32_t best = group_best<G, PEEL>(L);
+        const int32_t win = __shfl(L, (int)(best & 0xffu), G);
+        if (p == 0 && s.ok && s.oe != s.ob) {
+            next[s.v] = win;
             changed += (win != old);
         }
+    };
+    int64_t base = (int64_t)blockIdx.x * per_block;   // uniform round index
+    int64_t i = base + threadIdx.x / G;               // this lane's vertex index
+    // Four state slots rotate through the stages; the loop is unrolled by
+    // the rotation period so no register holding an in-flight load is ever
+    // copied (a copy would wait for it).
+    St sa = state(i), sb = state(i + stride), sc = state(i + 2 * stride), sd;
+    int32_t u = ldn(sa);
+    int32_t LA = gat(u, sa), oA = prev[sa.v], LB, oB;
+    u = ldn(sb);
+#define GALP_GROUP_STAGE(S0, S1, S2, S3, LC, OC, LN, ON) \
+    LN = gat(u, S1);                                     \
+    ON = prev[S1.v];                                     \
+    u = ldn(S2);                                         \
+    S3 = state(i + 3 * stride);                          \
+    resolve(S0, LC, OC);                                 \
+    i += stride;                                         \
+    base += stride;                                      \
+    if (base >= nv) break;
+    while (base < nv) {
+        GALP_GROUP_STAGE(sa, sb, sc, sd, LA, oA, LB, oB)
+        GALP_GROUP_STAGE(sb, sc, sd, sa, LB, oB, LA, oA)
+        GALP_GROUP_STAGE(sc, sd, sa, sb, LA, oA, LB, oB)
+        GALP_GROUP_STAGE(sd, sa, sb, sc, LB, oB, LA, oA)
     }
-    block_count_add(changed, counter, scratch);
+#undef GALP_GROUP_STAGE
+    block_count_add<4>(changed, counter, scratch);
+}
+
+// ---------------------------------------------------------------------------
+// Software pipeline shared by the table kernels.  A "work item" is a vertex
+// (LDS kernels) or a hub tile; Policy supplies
+//   struct Item { bool valid; int len; ... };
+//   Item item(int64_t i)               -- uniform per block (scalar loads)
+//   int64_t edge0(const Item &)        -- absolute index of the item's edge 0
+//   void process(const Item &, const int32_t (&lab)[EPT])
+// The driver keeps: labels of item t (being processed), neighbour ids of
+// item t+1, descriptor of item t+2; a period-2 unroll ping-pongs the label
+// registers so no in-flight load is ever copied.
+template <int T, int EPT, class Policy>
+__device__ __forceinline__ void pipelined(Policy &pol, const int32_t *__restrict__ nbr,
+                                          const int32_t *__restrict__ prev, int64_t n_items) {
+    using Item = typename Policy::Item;
+    const int64_t stride = gridDim.x;
+    int64_t i = blockIdx.x;
+    // Every lane issues every load (from a clamped, valid address) and
+    // selects afterwards: a load under a lane condition becomes a branch,
+    // and the compiler then drains all outstanding loads at the join.
+    // Loaded values are kept raw: masking them right after the load would
+    // wait for it.  Slots past an item's length hold junk that process()
+    // never reads (it checks pos < len).
+    auto load_nbr = [&](int32_t (&u)[EPT], const Item &it) {
+        const int64_t e0 = it.valid ? pol.edge0(it) : 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + (int)threadIdx.x;
+            const bool ok = it.valid && pos < it.len;
+            u[k] = __builtin_nontemporal_load(&nbr[ok ? e0 + pos : 0]);
+        }
+    };
+    auto gather = [&](int32_t (&lab)[EPT], const int32_t (&u)[EPT], const Item &it) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + (int)threadIdx.x;
+            const bool ok = it.valid && pos < it.len;
+            lab[k] = prev[ok ? u[k] : 0];
+        }
+    };
+    Item s0 = pol.item(i), s1 = pol.item(i + stride), s2 = pol.item(i + 2 * stride);
+    int32_t u[EPT], labA[EPT], labB[EPT];
+    load_nbr(u, s0);
+    gather(labA, u, s0);
+    load_nbr(u, s1);
+    while (s0.valid) {
+        gather(labB, u, s1);    // labels of item t+1
+        load_nbr(u, s2);        // neighbours of item t+2
+        Item s3 = pol.item(i + 3 * stride);
+        pol.process(s0, labA);  // item t
+        s0 = s1; s1 = s2; s2 = s3; i += stride;
+        if (!s0.valid) break;
+        gather(labA, u, s1);
+        load_nbr(u, s2);
+        s3 = pol.item(i + 3 * stride);
+        pol.process(s0, labB);
+        s0 = s1; s1 = s2; s2 = s3; i += stride;
+    }
+    (void)n_items;
+}
+
+// Open-addressing insert into an LDS table of 2^BITS slots; returns the slot
+// and sets *created when this lane claimed an empty slot.  The caller
+// guarantees fewer keys than slots; the probe bound only guards the GPU.
+template <int BITS>
+__device__ __forceinline__ uint32_t lds_insert(uint32_t *skey, uint32_t L, bool *created, int32_t *err) {
+    constexpr uint32_t MASK = (1u << BITS) - 1u;
+    uint32_t h = hash_slot(L, BITS);
+    *created = false;
+    for (uint32_t probe = 0; probe <= MASK; ++probe) {
+        const uint32_t cur = skey[h];
+        if (cur == L) return h;
+        if (cur == kEmpty) {
+            const uint32_t old = atomicCAS(&skey[h], kEmpty, L);
+            if (old == kEmpty) {
+                *created = true;
+                return h;
+            }
+            if (old == L) return h;
+        }
+        h = (h + 1) & MASK;
+    }
+    atomicOr(err, 1);
+    return h;
+}
+
+// Inserts one 64-lane chunk of labels (L < 0: no edge) into an LDS table,
+// adding each label's count (and, if LAST, the max of its positions `pos`).
+// Up to PEEL labels are committed by a single leader lane after a ballot --
+// in a converged neighbourhood one or two labels fill the whole chunk, and
+// 64 lanes hitting one counter would serialise -- then the remaining lanes
+// insert one by one.  Returns the lane's slot; *own marks lanes that wrote
+// the slot (they reset it afterwards), *creator the one lane per table that
+// claimed it; *maxc tracks the largest count seen.
+template <int BITS, int PEEL, bool LAST>
+__device__ __forceinline__ uint32_t lds_insert_chunk(uint32_t *skey, uint32_t *scnt, uint32_t *slast,
+                                                     int32_t L, uint32_t pos, bool *own,
+                                                     bool *creator, uint32_t *maxc, int32_t *err) {
+    const int lane = lane_id();
+    bool active = L >= 0;
+    uint32_t slot = 0;
+    *own = false;
+    *creator = false;
+#pragma unroll
+    for (int r = 0; r < PEEL; ++r) {
+        const unsigned long long ball = __ballot(active);
+        if (ball == 0) break;
+        const int leader = __ffsll((long long)ball) - 1;
+        const int32_t Lk = __builtin_amdgcn_readlane(L, leader);
+        const bool eq = active && (L == Lk);
+        const unsigned long long m = __ballot(eq);
+        const uint32_t lastpos = LAST ? (uint32_t)__builtin_amdgcn_readlane((int)pos, 63 - __clzll((long long)m)) : 0u;
+        uint32_t sl = 0;
+        if (lane == leader) {
+            bool created;
+            const uint32_t c = (uint32_t)__popcll(m);
+            sl = lds_insert<BITS>(skey, (uint32_t)Lk, &created, err);
+            *creator = created;
+            *maxc = max(*maxc, atomicAdd(&scnt[sl], c) + c);
+            if (LAST) atomicMax(&slast[sl], lastpos);
+            *own = true;
+        }
+        sl = (uint32_t)__builtin_amdgcn_readlane((int)sl, leader);
+        if (eq) slot = sl;
+        active = active && !eq;
+    }
+    if (active) {
+        bool created;
+        slot = lds_insert<BITS>(skey, (uint32_t)L, &created, err);
+        *creator = created;
+        *maxc = max(*maxc, atomicAdd(&scnt[slot], 1u) + 1u);
+        if (LAST) atomicMax(&slast[slot], pos);
+        *own = true;
+    }
+    return slot;
 }
 
 // ---------------------------------------------------------------------------
 // 64 < degree <= T*EPT: one block of T threads per vertex, LDS table of
-// 2^ceil(log2(2*deg)) slots (load factor <= 1/2).  Pass 1 inserts (CAS on the
-// key) and counts (atomic add); the block max of the returned counts is the
-// winning count C.  Pass 2 re-reads each edge's slot from registers and keeps
-// the largest position whose label has count C: that position's label is
-// argmax (count, last).
+// 2^CAPBITS = 2*T*EPT slots.  Pass 1 inserts (CAS on the key) and counts
+// (atomic add); the block max of the returned counts is the winning count C.
+// Pass 2 keeps the largest position whose label has count C: that
+// position's label is argmax (count, last).  Each lane then resets the slots
+// it used, leaving the table empty for the next vertex.
+template <int T, int EPT, int CAPBITS, typename E>
+struct LdsVertexPolicy {
+    struct Item {
+        bool valid;
+        int len;
+        int64_t v;
+        int64_t b;
+        int32_t old;
+    };
+    const E *off;
+    const int32_t *prev;
+    int32_t *next;
+    const int32_t *vlist;
+    int64_t v0, nv;
+    uint32_t *skey, *scnt, *sred;
+    unsigned long long *sred64;
+    int32_t *err;
+    uint32_t changed;
+
+    __device__ Item item(int64_t i) const {
+        Item it{false, 0, 0, 0, 0};
+        if (i < nv) {
+            it.v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
+            it.b = (int64_t)off[it.v];
+            it.len = (int)((int64_t)off[it.v + 1] - it.b);
+            it.old = prev[it.v];
+            it.valid = true;
+            if (it.len > T * EPT) {   // never planned into this class
+                if (threadIdx.x == 0) atomicOr(err, 8);
+                it.len = 0;
+            }
+        }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.b; }
+
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        if (it.len == 0) return;
+        uint32_t slot[EPT];
+        bool own[EPT];
+        uint32_t maxc = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + (int)threadIdx.x;
+            bool creator;
+            slot[k] = lds_insert_chunk<CAPBITS, 3, false>(skey, scnt, nullptr, pos < it.len ? lab[k] : -1,
+                                                          (uint32_t)pos, &own[k], &creator, &maxc, err);
+        }
+        __syncthreads();
+        maxc = block_max<T / kWave>(maxc, sred);
+        unsigned long long best = 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + (int)threadIdx.x;
+            if (pos < it.len && scnt[slot[k]] == maxc)
+                best = ((unsigned long long)(pos + 1) << 32) | (uint32_t)lab[k];
+        }
+        best = block_max<T / kWave>(best, sred64);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            if (own[k]) {
+                skey[slot[k]] = kEmpty;
+                scnt[slot[k]] = 0;
+            }
+        }
+        if (threadIdx.x == 0) {
+            const int32_t w = (int32_t)(uint32_t)best;
+            next[it.v] = w;
+            changed += (w != it.old);
+        }
+    }
+};
+
 template <int T, int EPT, int CAPBITS, typename E>
 __global__ void __launch_bounds__(T)
 k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
@@ -97,89 +388,141 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
            const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
            int32_t *__restrict__ err) {
+    static_assert((1 << CAPBITS) >= 2 * T * EPT, "load factor must stay <= 1/2");
     constexpr int CAP = 1 << CAPBITS;
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t sred[T / kWave];
     __shared__ unsigned long long sred64[T / kWave];
     if (*stop) return;
-    uint32_t changed = 0;
-    for (int64_t i = blockIdx.x; i < nv; i += gridDim.x) {
-        const int64_t v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
-        const E b = off[v], e = off[v + 1];
-        const int deg = (int)(e - b);
-        const int bits = max(6, ceil_log2_dev((uint32_t)(2 * deg)));
-        if (bits > CAPBITS || deg > T * EPT) {   // outside this class: never planned here
-            if (threadIdx.x == 0) atomicOr(err, 8);
-            continue;
-        }
-        const uint32_t mask = (1u << bits) - 1u;
-        for (int s = threadIdx.x; s <= (int)mask; s += T) {
-            skey[s] = kEmpty;
-            scnt[s] = 0;
-        }
-        __syncthreads();
-        int32_t lab[EPT];
-        uint32_t slot[EPT];
-        uint32_t maxc = 0;
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int pos = k * T + (int)threadIdx.x;
-            lab[k] = -1;
-            slot[k] = 0;
-            if (pos < deg) {
-                const uint32_t L = (uint32_t)prev[nbr[b + pos]];
-                lab[k] = (int32_t)L;
-                uint32_t h = hash_slot(L, bits);
-                for (uint32_t probe = 0;; ++probe) {
-                    if (probe > mask) {   // impossible: at most deg keys in 2*deg slots
-                        atomicOr(err, 1);
-                        break;
-                    }
-                    const uint32_t cur = skey[h];
-                    if (cur == L) break;
-                    if (cur == kEmpty) {
-                        const uint32_t old = atomicCAS(&skey[h], kEmpty, L);
-                        if (old == kEmpty || old == L) break;
-                    }
-                    h = (h + 1) & mask;
-                }
-                slot[k] = h;
-                const uint32_t c = atomicAdd(&scnt[h], 1u) + 1u;
-                maxc = max(maxc, c);
-            }
-        }
-        __syncthreads();
-        maxc = block_max(maxc, sred);
-        unsigned long long best = 0;
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int pos = k * T + (int)threadIdx.x;
-            if (pos < deg && scnt[slot[k]] == maxc)
-                best = ((unsigned long long)(pos + 1) << 32) | (uint32_t)lab[k];
-        }
-        best = block_max(best, sred64);
-        if (threadIdx.x == 0 && deg > 0) {
-            const int32_t w = (int32_t)(uint32_t)best;
-            const int32_t old = prev[v];
-            next[v] = w;
-            changed += (w != old);
-        }
-        __syncthreads();
+    for (int s = threadIdx.x; s < CAP; s += T) {
+        skey[s] = kEmpty;
+        scnt[s] = 0;
     }
-    block_count_add(changed, counter, sred);
+    LdsVertexPolicy<T, EPT, CAPBITS, E> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err, 0u};
+    pipelined<T, EPT>(pol, nbr, prev, nv);
+    __syncthreads();
+    block_count_add<T / kWave>(pol.changed, counter, sred);
 }
 
 // ---------------------------------------------------------------------------
 // Hubs: tile pass.  A block aggregates one tile of <= kHubTile edges in an
-// LDS table of (key, count, last position in the row), then merges each
-// occupied slot into the hub's global table with device-scope atomics.
+// LDS table of (key, count, last position in the row).  The lane that
+// created a slot then merges it into the hub's global table with
+// device-scope atomics (CAS on the key, add on the count, max on the last
+// position); lanes that create a global slot append it to the hub's list of
+// occupied slots (one atomic per wave).
+struct HubTilePolicyBase {};
+
+template <int T, int EPT, typename E>
+struct HubTilePolicy {
+    struct Item {
+        bool valid;
+        int len;
+        int hub;
+        int64_t e0;
+        int64_t rowb;
+    };
+    const E *off;
+    const HubTile *tiles;
+    int64_t ntiles;
+    const int64_t *hub_toff;
+    const int32_t *hub_tbits;
+    uint32_t *gkey, *gcnt, *glast;
+    int32_t *hub_nocc;
+    int32_t *occ;
+    uint32_t *skey, *scnt, *slast;
+    int32_t *err;
+
+    __device__ Item item(int64_t t) const {
+        Item it{false, 0, 0, 0, 0};
+        if (t < ntiles) {
+            const HubTile tl = tiles[t];
+            it.valid = true;
+            it.hub = tl.hub;
+            it.e0 = tl.e0;
+            it.len = (int)(tl.e1 - tl.e0);
+            it.rowb = (int64_t)off[tl.vtx];
+        }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.e0; }
+
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        uint32_t slot[EPT];
+        bool own[EPT], creator[EPT];
+        uint32_t maxc = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + (int)threadIdx.x;
+            slot[k] = lds_insert_chunk<kHubTileCapBits, 3, true>(
+                skey, scnt, slast, pos < it.len ? lab[k] : -1, (uint32_t)(it.e0 + pos - it.rowb), &own[k],
+                &creator[k], &maxc, err);
+        }
+        __syncthreads();
+        const int64_t gb = hub_toff[it.hub];
+        const int gbits = hub_tbits[it.hub];
+        const uint32_t gmask = (1u << gbits) - 1u;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            bool created = false;
+            uint32_t h = 0;
+            if (creator[k]) {   // exactly one lane per distinct label of the tile
+                const uint32_t L = (uint32_t)lab[k];
+                h = hash_slot(L, gbits);
+                for (uint32_t probe = 0;; ++probe) {
+                    if (probe > gmask) {
+                        atomicOr(err, 4);
+                        break;
+                    }
+                    const uint32_t cur = load_coherent(&gkey[gb + h]);
+                    if (cur == L) break;
+                    if (cur == kEmpty) {
+                        const uint32_t old = atomicCAS(&gkey[gb + h], kEmpty, L);
+                        if (old == kEmpty) {
+                            created = true;
+                            break;
+                        }
+                        if (old == L) break;
+                    }
+                    h = (h + 1) & gmask;
+                }
+                atomicAdd(&gcnt[gb + h], scnt[slot[k]]);
+                atomicMax(&glast[gb + h], slast[slot[k]]);
+            }
+            // append created slots to the hub's occupied list, one atomic per wave
+            const unsigned long long ball = __ballot(created);
+            if (ball) {
+                int base = 0;
+                const int leader = __ffsll((long long)ball) - 1;
+                if (lane_id() == leader) base = atomicAdd(&hub_nocc[it.hub], __popcll(ball));
+                base = __shfl(base, leader);
+                if (created) {
+                    const int rank = __popcll(ball & ((1ull << lane_id()) - 1ull));
+                    occ[gb + base + rank] = (int32_t)h;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            if (own[k]) {
+                skey[slot[k]] = kEmpty;
+                scnt[slot[k]] = 0;
+                slast[slot[k]] = 0;
+            }
+        }
+    }
+};
+
 template <int T, typename E>
 __global__ void __launch_bounds__(T)
 k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr,
            const int32_t *__restrict__ prev, const HubTile *__restrict__ tiles, int64_t ntiles,
            const int64_t *__restrict__ hub_toff, const int32_t *__restrict__ hub_tbits,
            uint32_t *__restrict__ gkey, uint32_t *__restrict__ gcnt, uint32_t *__restrict__ glast,
+           int32_t *__restrict__ hub_nocc, int32_t *__restrict__ occ,
            const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
@@ -187,80 +530,23 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr,
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t slast[CAP];
     if (*stop) return;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const HubTile tl = tiles[t];
-        const int64_t rowb = (int64_t)off[tl.vtx];
-        const int len = (int)(tl.e1 - tl.e0);
-        const int bits = max(6, ceil_log2_dev((uint32_t)(2 * len)));
-        if (bits > kHubTileCapBits || len > kHubTile) {
-            if (threadIdx.x == 0) atomicOr(err, 16);
-            continue;
-        }
-        const uint32_t mask = (1u << bits) - 1u;
-        for (int s = threadIdx.x; s <= (int)mask; s += T) {
-            skey[s] = kEmpty;
-            scnt[s] = 0;
-            slast[s] = 0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int pos = k * T + (int)threadIdx.x;
-            if (pos < len) {
-                const int64_t ea = tl.e0 + pos;
-                const uint32_t L = (uint32_t)prev[nbr[ea]];
-                uint32_t h = hash_slot(L, bits);
-                for (uint32_t probe = 0;; ++probe) {
-                    if (probe > mask) {
-                        atomicOr(err, 2);
-                        break;
-                    }
-                    const uint32_t cur = skey[h];
-                    if (cur == L) break;
-                    if (cur == kEmpty) {
-                        const uint32_t old = atomicCAS(&skey[h], kEmpty, L);
-                        if (old == kEmpty || old == L) break;
-                    }
-                    h = (h + 1) & mask;
-                }
-                atomicAdd(&scnt[h], 1u);
-                atomicMax(&slast[h], (uint32_t)(ea - rowb));
-            }
-        }
-        __syncthreads();
-        const int64_t gb = hub_toff[tl.hub];
-        const int gbits = hub_tbits[tl.hub];
-        const uint32_t gmask = (1u << gbits) - 1u;
-        for (int s = threadIdx.x; s <= (int)mask; s += T) {
-            const uint32_t L = skey[s];
-            if (L == kEmpty) continue;
-            uint32_t h = hash_slot(L, gbits);
-            for (uint32_t probe = 0;; ++probe) {
-                if (probe > gmask) {
-                    atomicOr(err, 4);
-                    break;
-                }
-                const uint32_t cur = load_coherent(&gkey[gb + h]);
-                if (cur == L) break;
-                if (cur == kEmpty) {
-                    const uint32_t old = atomicCAS(&gkey[gb + h], kEmpty, L);
-                    if (old == kEmpty || old == L) break;
-                }
-                h = (h + 1) & gmask;
-            }
-            atomicAdd(&gcnt[gb + h], scnt[s]);
-            atomicMax(&glast[gb + h], slast[s]);
-        }
-        __syncthreads();
+    for (int s = threadIdx.x; s < CAP; s += T) {
+        skey[s] = kEmpty;
+        scnt[s] = 0;
+        slast[s] = 0;
     }
+    HubTilePolicy<T, EPT, E> pol{off,  tiles, ntiles, hub_toff, hub_tbits, gkey, gcnt, glast,
+                                 hub_nocc, occ, skey, scnt,  slast,     err};
+    pipelined<T, EPT>(pol, nbr, prev, ntiles);
 }
 
-// Hubs: finalize.  One block per hub scans its global table for the max of
-// (count << 32 | last), writes the winning key, and clears the table.
+// Hubs: finalize.  One block per hub reduces the listed slots to the max of
+// (count << 32 | last), writes the winning key, clears the slots and the list.
 template <int T>
 __global__ void __launch_bounds__(T)
 k_hub_final(const int32_t *__restrict__ hub_vid, int64_t nhubs,
-            const int64_t *__restrict__ hub_toff, const int32_t *__restrict__ hub_tbits,
+            const int64_t *__restrict__ hub_toff, int32_t *__restrict__ hub_nocc,
+            const int32_t *__restrict__ occ,
             uint32_t *__restrict__ gkey, uint32_t *__restrict__ gcnt, uint32_t *__restrict__ glast,
             const int32_t *__restrict__ prev, int32_t *__restrict__ next,
             unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
@@ -271,27 +557,25 @@ k_hub_final(const int32_t *__restrict__ hub_vid, int64_t nhubs,
     uint32_t changed = 0;
     for (int64_t h = blockIdx.x; h < nhubs; h += gridDim.x) {
         const int64_t gb = hub_toff[h];
-        const int64_t cap = (int64_t)1 << hub_tbits[h];
+        const int nocc = hub_nocc[h];
         unsigned long long best = 0;
         uint32_t bestkey = 0;
-        for (int64_t s = threadIdx.x; s < cap; s += T) {
-            const uint32_t k = gkey[gb + s];
-            if (k != kEmpty) {
-                const unsigned long long pk =
-                    ((unsigned long long)gcnt[gb + s] << 32) | (unsigned long long)glast[gb + s];
-                if (pk > best) {
-                    best = pk;
-                    bestkey = k;
-                }
-                gkey[gb + s] = kEmpty;
-                gcnt[gb + s] = 0;
-                glast[gb + s] = 0;
+        for (int i = threadIdx.x; i < nocc; i += T) {
+            const int64_t s = gb + occ[gb + i];
+            const unsigned long long pk = ((unsigned long long)gcnt[s] << 32) | (unsigned long long)glast[s];
+            if (pk > best) {
+                best = pk;
+                bestkey = gkey[s];
             }
+            gkey[s] = kEmpty;
+            gcnt[s] = 0;
+            glast[s] = 0;
         }
-        const unsigned long long top = block_max(best, sred64);
+        const unsigned long long top = block_max<T / kWave>(best, sred64);
         if (best == top && top != 0) swin = bestkey;   // (count, last) is unique per label
         __syncthreads();
         if (threadIdx.x == 0) {
+            hub_nocc[h] = 0;
             const int32_t v = hub_vid[h];
             const int32_t w = (int32_t)swin;
             const int32_t old = prev[v];
@@ -300,7 +584,7 @@ k_hub_final(const int32_t *__restrict__ hub_vid, int64_t nhubs,
         }
         __syncthreads();
     }
-    block_count_add(changed, counter, sred);
+    block_count_add<T / kWave>(changed, counter, sred);
 }
 
 __global__ void k_iter_end(unsigned long long *counter, int32_t *stop, long long *iter_counts,
@@ -323,65 +607,71 @@ static int grid_for(const galp_graph *g, int64_t work_items, int blocks_per_cu) 
 }
 
 template <typename E>
+static double class_bytes(const BucketRange &r) {
+    // offsets, neighbour ids, gathered labels, own previous label, next label
+    return (double)sizeof(E) * (double)(r.nv + 1) + 8.0 * (double)r.edges + 8.0 * (double)r.nv;
+}
+
+template <int G, typename E>
+static void launch_group(galp_graph *g, int bid, const int32_t *prev, int32_t *next) {
+    const BucketRange &r = g->bucket[bid];
+    if (!r.nv) return;
+    PendingLaunch pl;
+    prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
+    constexpr int PEEL = G < 4 ? G : 4;
+    const int64_t per_block = 256 / G;
+    const int64_t blocks = (r.nv + per_block - 1) / per_block;
+    if (g->d_vlist)
+        k_hash_group<G, PEEL, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
+    else
+        k_hash_group<G, PEEL, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
+    prof_end(g, &pl);
+}
+
+template <int T, int EPT, int CAPBITS, typename E>
+static void launch_lds(galp_graph *g, int bid, const int32_t *prev, int32_t *next, int blocks_per_cu) {
+    const BucketRange &r = g->bucket[bid];
+    if (!r.nv) return;
+    PendingLaunch pl;
+    prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
+    k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+        (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop,
+        g->d_err);
+    prof_end(g, &pl);
+}
+
+template <typename E>
 static int hash_iteration(galp_graph *g) {
-    const E *off = (const E *)g->d_off;
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
-    hipStream_t s = g->stream;
-    PendingLaunch pl;
-    auto bytes = [&](const BucketRange &r) {
-        // offsets (two reads per vertex, one line-shared), neighbours, gathered
-        // labels, own previous label, next label
-        return (double)sizeof(E) * (double)(r.nv + 1) + 8.0 * (double)r.edges + 8.0 * (double)r.nv;
-    };
-#define GALP_GROUP(G, BID)                                                                      \
-    {                                                                                           \
-        const BucketRange &r = g->bucket[BID];                                                  \
-        if (r.nv) {                                                                             \
-            prof_begin(g, "hash_group" #G, bytes(r), &pl);                                      \
-            const int64_t per_block = 256 / G;                                                  \
-            k_hash_group<G, E><<<grid_for(g, (r.nv + per_block - 1) / per_block, 16), 256, 0, s>>>( \
-                off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);    \
-            prof_end(g, &pl);                                                                   \
-        }                                                                                       \
-    }
-    GALP_GROUP(4, B_G4)
-    GALP_GROUP(8, B_G8)
-    GALP_GROUP(16, B_G16)
-    GALP_GROUP(32, B_G32)
-    GALP_GROUP(64, B_G64)
-#undef GALP_GROUP
-    {
-        const BucketRange &r = g->bucket[B_WAVE];
-        if (r.nv) {
-            prof_begin(g, "hash_wave", bytes(r), &pl);
-            k_hash_lds<64, 8, 10, E><<<grid_for(g, r.nv, 20), 64, 0, s>>>(
-                off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
-            prof_end(g, &pl);
-        }
-    }
-    {
-        const BucketRange &r = g->bucket[B_BLOCK];
-        if (r.nv) {
-            prof_begin(g, "hash_block", bytes(r), &pl);
-            k_hash_lds<256, 16, 13, E><<<grid_for(g, r.nv, 2), 256, 0, s>>>(
-                off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
-            prof_end(g, &pl);
-        }
-    }
+    // largest work first
     if (g->nhubs) {
         const BucketRange &r = g->bucket[B_HUB];
-        prof_begin(g, "hash_hub_tile", 4.0 * (double)r.edges * 2.0, &pl);
-        k_hub_tile<256, E><<<grid_for(g, g->ntiles, 2), 256, 0, s>>>(
-            off, g->d_nbr, prev, g->d_tiles, g->ntiles, g->d_hub_toff, g->d_hub_tbits, g->d_gkey,
-            g->d_gcnt, g->d_glast, g->d_stop, g->d_err);
+        PendingLaunch pl;
+        prof_begin(g, "hub_tile", 8.0 * (double)r.edges, &pl);
+        k_hub_tile<256, E><<<grid_for(g, g->ntiles, 2), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, g->d_tiles, g->ntiles, g->d_hub_toff, g->d_hub_tbits,
+            g->d_gkey, g->d_gcnt, g->d_glast, g->d_hub_nocc, g->d_occ, g->d_stop, g->d_err);
         prof_end(g, &pl);
-        prof_begin(g, "hash_hub_final", 16.0 * (double)r.nv, &pl);
-        k_hub_final<1024><<<grid_for(g, g->nhubs, 2), 1024, 0, s>>>(
-            g->d_hub_vid, g->nhubs, g->d_hub_toff, g->d_hub_tbits, g->d_gkey, g->d_gcnt, g->d_glast,
-            prev, next, g->d_counter, g->d_stop);
+        prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)r.nv + 8.0 * (double)r.nv, &pl);
+        k_hub_final<1024><<<grid_for(g, g->nhubs, 2), 1024, 0, g->stream>>>(
+            g->d_hub_vid, g->nhubs, g->d_hub_toff, g->d_hub_nocc, g->d_occ, g->d_gkey, g->d_gcnt,
+            g->d_glast, prev, next, g->d_counter, g->d_stop);
         prof_end(g, &pl);
     }
+    launch_lds<512, 8, 13, E>(g, B_B4K, prev, next, 2);
+    launch_lds<256, 8, 12, E>(g, B_B2K, prev, next, 4);
+    launch_lds<128, 8, 11, E>(g, B_B1K, prev, next, 8);
+    launch_lds<64, 8, 10, E>(g, B_W512, prev, next, 16);
+    launch_lds<64, 4, 9, E>(g, B_W256, prev, next, 24);
+    launch_lds<64, 2, 8, E>(g, B_W128, prev, next, 32);
+    launch_group<64, E>(g, B_G64, prev, next);
+    launch_group<32, E>(g, B_G32, prev, next);
+    launch_group<16, E>(g, B_G16, prev, next);
+    launch_group<8, E>(g, B_G8, prev, next);
+    launch_group<4, E>(g, B_G4, prev, next);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
     return GALP_OK;
