@@ -125,9 +125,9 @@ static int dev_alloc(galp_graph *g, T **p, int64_t count) {
 
 static void dev_free_all(galp_graph *g) {
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
-                    g->d_out, g->d_tiles, g->d_hub_toff, g->d_hub_tbits, g->d_hub_vid, g->d_gkey,
-                    g->d_gcnt, g->d_glast, g->d_hub_nocc, g->d_occ, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts,
-                    g->d_err};
+                    g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
+                    g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
+                    g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts, g->d_err};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (g->h_pinned) (void)hipHostFree(g->h_pinned);
@@ -266,29 +266,42 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     }
     const std::vector<int64_t> &offi = g->reorder ? off_int : off;
 
-    // hubs: tiles and global tables
-    std::vector<HubTile> tiles;
-    std::vector<int64_t> hub_toff;
-    std::vector<int32_t> hub_tbits;
-    int64_t gcap = 0;
+    // hubs: phase-1 tiles and phase-2 (hub, label partition) items
+    std::vector<int32_t> hub_nparts, item_hub, item_part, hub_tile0, hub_ntiles, tile_hub;
+    std::vector<int64_t> tile_idx_off;
+    int64_t ntile_idx = 0;
     for (size_t h = 0; h < hub_vertices.size(); ++h) {
         const int32_t x = hub_vertices[h];
-        const int64_t b = offi[(size_t)x], e = offi[(size_t)x + 1];
-        for (int64_t t = b; t < e; t += kHubTile) tiles.push_back(HubTile{(int32_t)h, x, t, std::min(e, t + kHubTile)});
-        const int bits = std::max(6, ceil_log2_host(2 * (e - b)));
-        hub_toff.push_back(gcap);
-        hub_tbits.push_back(bits);
-        gcap += (int64_t)1 << bits;
+        const int64_t d = offi[(size_t)x + 1] - offi[(size_t)x];
+        if (d > (int64_t)kHubPart * kHubMaxParts)
+            return fail(ctx, GALP_EUNSUPPORTED, "a vertex of degree " + std::to_string(d) +
+                                                    " exceeds the hub path's limit of " +
+                                                    std::to_string((int64_t)kHubPart * kHubMaxParts));
+        const int32_t P = (int32_t)((d + kHubPart - 1) / kHubPart);
+        const int32_t nt = (int32_t)((d + kHubTile - 1) / kHubTile);
+        hub_nparts.push_back(P);
+        hub_tile0.push_back((int32_t)tile_hub.size());
+        hub_ntiles.push_back(nt);
+        for (int32_t t = 0; t < nt; ++t) {
+            tile_hub.push_back((int32_t)h);
+            tile_idx_off.push_back(ntile_idx);
+            ntile_idx += P + 1;
+        }
+        for (int32_t p = 0; p < P; ++p) {
+            item_hub.push_back((int32_t)h);
+            item_part.push_back(p);
+        }
     }
     g->nhubs = (int64_t)hub_vertices.size();
-    g->ntiles = (int64_t)tiles.size();
-    g->gcap = gcap;
+    g->nhub_items = (int64_t)item_hub.size();
+    g->nhub_tiles = (int64_t)tile_hub.size();
+    g->ntile_idx = ntile_idx;
 
     // ---- memory check (check_gpu_mem, main.cu:51-64) --------------------
     {
         const int64_t esz = g->off64 ? 8 : 4;
-        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 16 * gcap + (int64_t)sizeof(HubTile) * g->ntiles +
-                       16 * g->nhubs + 4096;
+        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 8 * g->nhub_items + 32 * g->nhubs +
+                       (12 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx + 4096;
         if (g->reorder) need += 8 * n + 4 * m + 8 * (n + 1);  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;
@@ -347,24 +360,31 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     }
 
     if (g->nhubs) {
-        std::vector<int32_t> hv(hub_vertices.begin(), hub_vertices.end());
-        GALP_TRY(dev_alloc(g, &g->d_tiles, g->ntiles));
-        GALP_TRY(dev_alloc(g, &g->d_hub_toff, g->nhubs));
-        GALP_TRY(dev_alloc(g, &g->d_hub_tbits, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_vid, g->nhubs));
-        GALP_TRY(dev_alloc(g, &g->d_gkey, gcap));
-        GALP_TRY(dev_alloc(g, &g->d_gcnt, gcap));
-        GALP_TRY(dev_alloc(g, &g->d_glast, gcap));
-        GALP_TRY(dev_alloc(g, &g->d_occ, gcap));
-        GALP_TRY(dev_alloc(g, &g->d_hub_nocc, g->nhubs));
-        GALP_HIP(ctx, hipMemcpy(g->d_tiles, tiles.data(), sizeof(HubTile) * tiles.size(), hipMemcpyHostToDevice));
-        GALP_HIP(ctx, hipMemcpy(g->d_hub_toff, hub_toff.data(), 8 * hub_toff.size(), hipMemcpyHostToDevice));
-        GALP_HIP(ctx, hipMemcpy(g->d_hub_tbits, hub_tbits.data(), 4 * hub_tbits.size(), hipMemcpyHostToDevice));
-        GALP_HIP(ctx, hipMemcpy(g->d_hub_vid, hv.data(), 4 * hv.size(), hipMemcpyHostToDevice));
-        GALP_TRY(dev_fill_u32(g, g->d_gkey, gcap, 0xffffffffu));
-        GALP_TRY(dev_fill_u32(g, g->d_gcnt, gcap, 0u));
-        GALP_TRY(dev_fill_u32(g, g->d_glast, gcap, 0u));
-        GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_nocc, g->nhubs, 0u));
+        GALP_TRY(dev_alloc(g, &g->d_hub_nparts, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_hub_best, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_hub_tile0, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_hub_ntiles, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_item_hub, g->nhub_items));
+        GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
+        GALP_TRY(dev_alloc(g, &g->d_tile_hub, g->nhub_tiles));
+        GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
+        GALP_TRY(dev_alloc(g, &g->d_tile_idx, ntile_idx));
+        GALP_TRY(dev_alloc(g, &g->d_out_key, g->nhub_tiles * kHubTile));
+        GALP_TRY(dev_alloc(g, &g->d_out_cnt, g->nhub_tiles * kHubTile));
+        GALP_TRY(dev_alloc(g, &g->d_out_last, g->nhub_tiles * kHubTile));
+        auto up = [&](void *dst, const void *src, size_t bytes) {
+            return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+        };
+        GALP_HIP(ctx, up(g->d_hub_vid, hub_vertices.data(), 4 * hub_vertices.size()));
+        GALP_HIP(ctx, up(g->d_hub_nparts, hub_nparts.data(), 4 * hub_nparts.size()));
+        GALP_HIP(ctx, up(g->d_hub_tile0, hub_tile0.data(), 4 * hub_tile0.size()));
+        GALP_HIP(ctx, up(g->d_hub_ntiles, hub_ntiles.data(), 4 * hub_ntiles.size()));
+        GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
+        GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
+        GALP_HIP(ctx, up(g->d_tile_hub, tile_hub.data(), 4 * tile_hub.size()));
+        GALP_HIP(ctx, up(g->d_tile_idx_off, tile_idx_off.data(), 8 * tile_idx_off.size()));
+        GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
     }
     GALP_HIP(ctx, hipStreamSynchronize(g->stream));
     g->preprocess_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -26,14 +26,17 @@ enum BucketId {
     B_B1K,      // 513..1024  128 threads, 2048 slots
     B_B2K,      // 1025..2048 256 threads, 4096 slots
     B_B4K,      // 2049..4096 512 threads, 8192 slots
-    B_HUB,      // > 4096     2048-edge tiles merged in a global table
+    B_HUB,      // > 4096     label-partitioned blocks, ceil(deg/2048) per hub
     NUM_BUCKETS
 };
 extern const int64_t kBucketHi[NUM_BUCKETS];
 extern const char *const kBucketName[NUM_BUCKETS];
 
-constexpr int kHubTile = 2048;        // edges per hub tile
-constexpr int kHubTileCapBits = 12;   // LDS table of a hub tile (4096 slots)
+constexpr int kHubTile = 2048;        // edges per hub tile (phase 1)
+constexpr int kHubTileCapBits = 12;   // LDS table of a tile (4096 slots)
+constexpr int kHubPart = 2048;        // hub partitions: ceil(deg / kHubPart)
+constexpr int kHubMaxParts = 4096;    // so hub degree <= kHubPart * kHubMaxParts
+constexpr int kHubMergeCapBits = 12;  // LDS table of one partition (4096 slots)
 
 struct BucketRange {
     int64_t v0 = 0;      // first renumbered id, or first index into vlist
@@ -41,11 +44,6 @@ struct BucketRange {
     int64_t edges = 0;   // sum of their degrees
 };
 
-struct HubTile {
-    int32_t hub;         // index into the hub arrays
-    int32_t vtx;         // vertex id (internal numbering)
-    int64_t e0, e1;      // absolute edge range [e0, e1)
-};
 
 struct KernelAgg {
     std::string name;
@@ -96,15 +94,19 @@ struct galp_graph {
 
     galp::BucketRange bucket[galp::NUM_BUCKETS];
 
-    // hub path
-    int64_t nhubs = 0, ntiles = 0, gcap = 0;
-    galp::HubTile *d_tiles = nullptr;
-    int64_t *d_hub_toff = nullptr;  // table offset per hub
-    int32_t *d_hub_tbits = nullptr; // log2 table capacity per hub
-    int32_t *d_hub_vid = nullptr;   // vertex id per hub
-    uint32_t *d_gkey = nullptr, *d_gcnt = nullptr, *d_glast = nullptr;
-    int32_t *d_hub_nocc = nullptr;  // occupied slots per hub table
-    int32_t *d_occ = nullptr;       // occupied slot list, same layout as the tables
+    // hub path: phase-1 tiles, phase-2 items (hub, label partition)
+    int64_t nhubs = 0, nhub_items = 0, nhub_tiles = 0, ntile_idx = 0;
+    int32_t *d_hub_vid = nullptr;           // vertex id per hub
+    int32_t *d_hub_nparts = nullptr;        // partitions per hub
+    int32_t *d_hub_tile0 = nullptr;         // first tile of each hub
+    int32_t *d_hub_ntiles = nullptr;        // tiles of each hub
+    int32_t *d_tile_hub = nullptr;          // hub of each tile
+    int64_t *d_tile_idx_off = nullptr;      // offset of each tile's P_h + 1 partition offsets
+    int32_t *d_tile_idx = nullptr;          // partition offsets of every tile
+    uint32_t *d_out_key = nullptr, *d_out_cnt = nullptr, *d_out_last = nullptr;  // kHubTile per tile
+    int32_t *d_item_hub = nullptr;          // hub of each merge item
+    int32_t *d_item_part = nullptr;         // partition of each merge item
+    unsigned long long *d_hub_best = nullptr;  // max (count << 32 | last) per hub
 
     // iteration control (device)
     unsigned long long *d_counter = nullptr;

@@ -12,9 +12,9 @@
 //     last positions come from lane exchanges, no table at all;
 //   - 65..4096: one wave (or one workgroup of up to 512 threads) per vertex
 //     with an open-addressing table in LDS at load factor <= 1/2;
-//   - hubs (> 4096): 2048-edge tiles aggregate in LDS, then merge (count add,
-//     last max) into a per-hub global table whose occupied slots are listed;
-//     a finalize kernel reduces only the listed slots and clears them.
+//   - hubs (> 4096): ceil(deg/2048) blocks per hub, each counting one hash
+//     partition of the labels in LDS over the whole row; a 64-bit atomicMax
+//     of (count, last) per block picks the winner, no global table at all.
 // Table-based kernels run a software pipeline over their vertices: the
 // labels of the next vertex and the neighbour ids of the one after are in
 // flight while the current vertex is counted, so the dependent
@@ -406,43 +406,56 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
 }
 
 // ---------------------------------------------------------------------------
-// Hubs: tile pass.  A block aggregates one tile of <= kHubTile edges in an
-// LDS table of (key, count, last position in the row).  The lane that
-// created a slot then merges it into the hub's global table with
-// device-scope atomics (CAS on the key, add on the count, max on the last
-// position); lanes that create a global slot append it to the hub's list of
-// occupied slots (one atomic per wave).
-struct HubTilePolicyBase {};
+// Hubs (degree > 4096): two phases, every edge gathered once.
+//  1. tile: a block aggregates one kHubTile-edge tile of a hub row in LDS
+//     (key, count, last position in the row), counting-sorts the tile's
+//     distinct labels by label partition (P_h = ceil(deg / kHubPart)
+//     partitions of a hash of the label) and writes them as a compact list
+//     plus P_h + 1 partition offsets;
+//  2. merge: block (h, p) folds partition p of every tile of hub h into an
+//     LDS table -- no two blocks share a label, so there is no device-scope
+//     table and no atomics on it -- and applies the tie rule within the
+//     partition; one 64-bit atomicMax of (count << 32 | last) per block
+//     picks the hub's winner, which is the label at position `last`.
+__device__ __forceinline__ uint32_t hub_partition(uint32_t L, uint32_t P) {
+    return (uint32_t)(((unsigned long long)(L * 0x85EBCA77u) * P) >> 32);
+}
 
 template <int T, int EPT, typename E>
 struct HubTilePolicy {
     struct Item {
         bool valid;
         int len;
-        int hub;
-        int64_t e0;
-        int64_t rowb;
+        int64_t e0;       // absolute index of the tile's first edge
+        uint32_t row0;    // position of that edge in its row
+        uint32_t P;       // partitions of the hub
+        int64_t idx;      // offset of the tile's partition offsets
+        int64_t t;        // tile id
     };
     const E *off;
-    const HubTile *tiles;
+    const int32_t *tile_hub;
+    const int32_t *hub_tile0;
+    const int32_t *hub_vid;
+    const int32_t *hub_nparts;
+    const int64_t *tile_idx_off;
     int64_t ntiles;
-    const int64_t *hub_toff;
-    const int32_t *hub_tbits;
-    uint32_t *gkey, *gcnt, *glast;
-    int32_t *hub_nocc;
-    int32_t *occ;
-    uint32_t *skey, *scnt, *slast;
+    uint32_t *skey, *scnt, *slast, *hist;
+    uint32_t *out_key, *out_cnt, *out_last;
+    int32_t *tile_idx;
     int32_t *err;
 
     __device__ Item item(int64_t t) const {
-        Item it{false, 0, 0, 0, 0};
+        Item it{false, 0, 0, 0, 1, 0, t};
         if (t < ntiles) {
-            const HubTile tl = tiles[t];
+            const int h = tile_hub[t];
+            const int64_t v = hub_vid[h];
+            const int64_t rb = (int64_t)off[v], re = (int64_t)off[v + 1];
+            it.row0 = (uint32_t)((t - (int64_t)hub_tile0[h]) * (int64_t)kHubTile);   // tiles of a hub are consecutive
+            it.e0 = rb + it.row0;
+            it.len = (int)min((int64_t)kHubTile, re - it.e0);
+            it.P = (uint32_t)hub_nparts[h];
+            it.idx = tile_idx_off[t];
             it.valid = true;
-            it.hub = tl.hub;
-            it.e0 = tl.e0;
-            it.len = (int)(tl.e1 - tl.e0);
-            it.rowb = (int64_t)off[tl.vtx];
         }
         return it;
     }
@@ -452,139 +465,184 @@ struct HubTilePolicy {
         uint32_t slot[EPT];
         bool own[EPT], creator[EPT];
         uint32_t maxc = 0;
+        for (uint32_t p = threadIdx.x; p <= it.P; p += T) hist[p] = 0;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * T + (int)threadIdx.x;
-            slot[k] = lds_insert_chunk<kHubTileCapBits, 3, true>(
-                skey, scnt, slast, pos < it.len ? lab[k] : -1, (uint32_t)(it.e0 + pos - it.rowb), &own[k],
-                &creator[k], &maxc, err);
+            slot[k] = lds_insert_chunk<kHubTileCapBits, 3, true>(skey, scnt, slast, pos < it.len ? lab[k] : -1,
+                                                                 it.row0 + (uint32_t)pos, &own[k], &creator[k],
+                                                                 &maxc, err);
         }
         __syncthreads();
-        const int64_t gb = hub_toff[it.hub];
-        const int gbits = hub_tbits[it.hub];
-        const uint32_t gmask = (1u << gbits) - 1u;
+        // counting sort of the tile's distinct labels by partition
+        uint32_t part[EPT];
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            bool created = false;
-            uint32_t h = 0;
-            if (creator[k]) {   // exactly one lane per distinct label of the tile
-                const uint32_t L = (uint32_t)lab[k];
-                h = hash_slot(L, gbits);
-                for (uint32_t probe = 0;; ++probe) {
-                    if (probe > gmask) {
-                        atomicOr(err, 4);
-                        break;
-                    }
-                    const uint32_t cur = load_coherent(&gkey[gb + h]);
-                    if (cur == L) break;
-                    if (cur == kEmpty) {
-                        const uint32_t old = atomicCAS(&gkey[gb + h], kEmpty, L);
-                        if (old == kEmpty) {
-                            created = true;
-                            break;
-                        }
-                        if (old == L) break;
-                    }
-                    h = (h + 1) & gmask;
-                }
-                atomicAdd(&gcnt[gb + h], scnt[slot[k]]);
-                atomicMax(&glast[gb + h], slast[slot[k]]);
+        for (int k = 0; k < EPT; ++k)
+            if (creator[k]) {
+                part[k] = hub_partition(skey[slot[k]], it.P);
+                atomicAdd(&hist[part[k] + 1], 1u);
             }
-            // append created slots to the hub's occupied list, one atomic per wave
-            const unsigned long long ball = __ballot(created);
-            if (ball) {
-                int base = 0;
-                const int leader = __ffsll((long long)ball) - 1;
-                if (lane_id() == leader) base = atomicAdd(&hub_nocc[it.hub], __popcll(ball));
-                base = __shfl(base, leader);
-                if (created) {
-                    const int rank = __popcll(ball & ((1ull << lane_id()) - 1ull));
-                    occ[gb + base + rank] = (int32_t)h;
-                }
+        __syncthreads();
+        if (threadIdx.x == 0) {   // P_h + 1 offsets; P_h is small next to the tile
+            uint32_t acc = 0;
+            for (uint32_t p = 0; p <= it.P; ++p) {
+                acc += hist[p];
+                hist[p] = acc;
+                tile_idx[it.idx + p] = (int32_t)acc;
             }
         }
         __syncthreads();
+        const int64_t ob = it.t * (int64_t)kHubTile;
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
+        for (int k = 0; k < EPT; ++k)
+            if (creator[k]) {
+                const uint32_t q = atomicAdd(&hist[part[k]], 1u);
+                out_key[ob + q] = skey[slot[k]];
+                out_cnt[ob + q] = scnt[slot[k]];
+                out_last[ob + q] = slast[slot[k]];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
             if (own[k]) {
                 skey[slot[k]] = kEmpty;
                 scnt[slot[k]] = 0;
                 slast[slot[k]] = 0;
             }
-        }
     }
 };
 
 template <int T, typename E>
 __global__ void __launch_bounds__(T)
-k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr,
-           const int32_t *__restrict__ prev, const HubTile *__restrict__ tiles, int64_t ntiles,
-           const int64_t *__restrict__ hub_toff, const int32_t *__restrict__ hub_tbits,
-           uint32_t *__restrict__ gkey, uint32_t *__restrict__ gcnt, uint32_t *__restrict__ glast,
-           int32_t *__restrict__ hub_nocc, int32_t *__restrict__ occ,
-           const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+           const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
+           const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
+           const int64_t *__restrict__ tile_idx_off, int64_t ntiles,
+           uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last,
+           int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t slast[CAP];
+    __shared__ uint32_t hist[kHubMaxParts + 1];
     if (*stop) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
         scnt[s] = 0;
         slast[s] = 0;
     }
-    HubTilePolicy<T, EPT, E> pol{off,  tiles, ntiles, hub_toff, hub_tbits, gkey, gcnt, glast,
-                                 hub_nocc, occ, skey, scnt,  slast,     err};
+    HubTilePolicy<T, EPT, E> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles, skey,
+                                 scnt, slast,    hist,      out_key,  out_cnt,    out_last,     tile_idx, err};
     pipelined<T, EPT>(pol, nbr, prev, ntiles);
 }
 
-// Hubs: finalize.  One block per hub reduces the listed slots to the max of
-// (count << 32 | last), writes the winning key, clears the slots and the list.
 template <int T>
 __global__ void __launch_bounds__(T)
-k_hub_final(const int32_t *__restrict__ hub_vid, int64_t nhubs,
-            const int64_t *__restrict__ hub_toff, int32_t *__restrict__ hub_nocc,
-            const int32_t *__restrict__ occ,
-            uint32_t *__restrict__ gkey, uint32_t *__restrict__ gcnt, uint32_t *__restrict__ glast,
-            const int32_t *__restrict__ prev, int32_t *__restrict__ next,
-            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
+k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t nitems,
+            const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
+            const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
+            const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_cnt,
+            const uint32_t *__restrict__ out_last, unsigned long long *__restrict__ hub_best,
+            const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+    constexpr int CAP = 1 << kHubMergeCapBits;
+    __shared__ uint32_t skey[CAP];
+    __shared__ uint32_t scnt[CAP];
+    __shared__ uint32_t slast[CAP];
+    __shared__ uint32_t slist[CAP];
+    __shared__ uint32_t nlist;
+    __shared__ int32_t slo[T], shi[T];
     __shared__ unsigned long long sred64[T / kWave];
-    __shared__ uint32_t sred[T / kWave];
-    __shared__ uint32_t swin;
+    if (*stop) return;
+    for (int s = threadIdx.x; s < CAP; s += T) {
+        skey[s] = kEmpty;
+        scnt[s] = 0;
+        slast[s] = 0;
+    }
+    if (threadIdx.x == 0) nlist = 0;
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int h = item_hub[it];
+        const int p = item_part[it];
+        const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        // tiles in chunks of T: the chunk's slice bounds are loaded one tile
+        // per thread (no chain of dependent index loads), then the block
+        // strides over each slice's entries
+        for (int64_t c0 = t0; c0 < t0 + nt; c0 += T) {
+            __syncthreads();
+            const int64_t t = c0 + threadIdx.x;
+            if (t < t0 + nt) {
+                const int64_t ib = tile_idx_off[t];
+                slo[threadIdx.x] = tile_idx[ib + p];
+                shi[threadIdx.x] = tile_idx[ib + p + 1];
+            }
+            __syncthreads();
+            const int nc = (int)min((int64_t)T, t0 + nt - c0);
+            for (int j = 0; j < nc; ++j) {
+                const int lo = slo[j], hi = shi[j];
+                for (int i0 = lo; i0 < hi; i0 += T) {   // uniform trip count: ballots below
+                    const int i = i0 + (int)threadIdx.x;
+                    bool created = false;
+                    uint32_t sl = 0;
+                    if (i < hi) {
+                        const int64_t e = (c0 + j) * (int64_t)kHubTile + i;
+                        sl = lds_insert<kHubMergeCapBits>(skey, out_key[e], &created, err);
+                        atomicAdd(&scnt[sl], out_cnt[e]);
+                        atomicMax(&slast[sl], out_last[e]);
+                    }
+                    // one list append per wave
+                    const unsigned long long ball = __ballot(created);
+                    if (ball) {
+                        const int leader = __ffsll((long long)ball) - 1;
+                        uint32_t base = 0;
+                        if (lane_id() == leader) base = atomicAdd(&nlist, (uint32_t)__popcll(ball));
+                        base = (uint32_t)__shfl((int)base, leader);
+                        if (created) slist[base + __popcll(ball & ((1ull << lane_id()) - 1ull))] = sl;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // argmax and reset over the slots created (cost follows the number of
+        // distinct labels, not the table size)
+        const uint32_t nl = nlist;
+        unsigned long long best = 0;
+        for (uint32_t i = threadIdx.x; i < nl; i += T) {
+            const uint32_t sl = slist[i];
+            const unsigned long long pk = ((unsigned long long)scnt[sl] << 32) | slast[sl];
+            best = pk > best ? pk : best;
+            skey[sl] = kEmpty;
+            scnt[sl] = 0;
+            slast[sl] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) nlist = 0;
+        best = block_max<T / kWave>(best, sred64);
+        if (threadIdx.x == 0 && best) atomicMax(&hub_best[h], best);
+    }
+}
+
+// Hubs: finalize.  The winning label is the one at position `last` of the
+// row; resets hub_best for the next iteration.
+template <typename E>
+__global__ void __launch_bounds__(256)
+k_hub_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ hub_vid,
+            int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ prev,
+            int32_t *__restrict__ next, unsigned long long *__restrict__ counter,
+            const int32_t *__restrict__ stop) {
+    __shared__ uint32_t sred[256 / kWave];
     if (*stop) return;
     uint32_t changed = 0;
-    for (int64_t h = blockIdx.x; h < nhubs; h += gridDim.x) {
-        const int64_t gb = hub_toff[h];
-        const int nocc = hub_nocc[h];
-        unsigned long long best = 0;
-        uint32_t bestkey = 0;
-        for (int i = threadIdx.x; i < nocc; i += T) {
-            const int64_t s = gb + occ[gb + i];
-            const unsigned long long pk = ((unsigned long long)gcnt[s] << 32) | (unsigned long long)glast[s];
-            if (pk > best) {
-                best = pk;
-                bestkey = gkey[s];
-            }
-            gkey[s] = kEmpty;
-            gcnt[s] = 0;
-            glast[s] = 0;
-        }
-        const unsigned long long top = block_max<T / kWave>(best, sred64);
-        if (best == top && top != 0) swin = bestkey;   // (count, last) is unique per label
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            hub_nocc[h] = 0;
-            const int32_t v = hub_vid[h];
-            const int32_t w = (int32_t)swin;
-            const int32_t old = prev[v];
-            next[v] = w;
-            changed += (w != old);
-        }
-        __syncthreads();
+    for (int64_t h = (int64_t)blockIdx.x * 256 + threadIdx.x; h < nhubs; h += (int64_t)gridDim.x * 256) {
+        const int64_t v = hub_vid[h];
+        const unsigned long long pk = hub_best[h];
+        const uint32_t last = (uint32_t)pk;
+        const int32_t w = prev[nbr[(int64_t)off[v] + last]];
+        next[v] = w;
+        changed += (w != prev[v]);
+        hub_best[h] = 0ull;
     }
-    block_count_add<T / kWave>(changed, counter, sred);
+    block_count_add<4>(changed, counter, sred);
 }
 
 __global__ void k_iter_end(unsigned long long *counter, int32_t *stop, long long *iter_counts,
@@ -651,14 +709,19 @@ static int hash_iteration(galp_graph *g) {
         const BucketRange &r = g->bucket[B_HUB];
         PendingLaunch pl;
         prof_begin(g, "hub_tile", 8.0 * (double)r.edges, &pl);
-        k_hub_tile<256, E><<<grid_for(g, g->ntiles, 2), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, g->d_tiles, g->ntiles, g->d_hub_toff, g->d_hub_tbits,
-            g->d_gkey, g->d_gcnt, g->d_glast, g->d_hub_nocc, g->d_occ, g->d_stop, g->d_err);
+        k_hub_tile<256, E><<<grid_for(g, g->nhub_tiles, 2), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
+            g->d_tile_idx_off, g->nhub_tiles, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop, g->d_err);
         prof_end(g, &pl);
-        prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)r.nv + 8.0 * (double)r.nv, &pl);
-        k_hub_final<1024><<<grid_for(g, g->nhubs, 2), 1024, 0, g->stream>>>(
-            g->d_hub_vid, g->nhubs, g->d_hub_toff, g->d_hub_nocc, g->d_occ, g->d_gkey, g->d_gcnt,
-            g->d_glast, prev, next, g->d_counter, g->d_stop);
+        prof_begin(g, "hub_merge", 0.0, &pl);
+        k_hub_merge<256><<<grid_for(g, g->nhub_items, 3), 256, 0, g->stream>>>(
+            g->d_item_hub, g->d_item_part, g->nhub_items, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
+            g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err);
+        prof_end(g, &pl);
+        prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)r.nv + 16.0 * (double)r.nv, &pl);
+        k_hub_final<E><<<grid_for(g, (g->nhubs + 255) / 256, 4), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, prev, next, g->d_counter,
+            g->d_stop);
         prof_end(g, &pl);
     }
     launch_lds<512, 8, 13, E>(g, B_B4K, prev, next, 2);
