@@ -119,6 +119,17 @@ def lib():
         "galp_host_last_error": (ctypes.c_char_p, []),
         "galp_accuracy": (ctypes.c_int, [ctypes.c_char_p, vp, i64, ctypes.POINTER(dbl), ctypes.POINTER(dbl),
                                          ctypes.POINTER(dbl)]),
+        "galp_mlp_num_params": (i64, [i32]),
+        "galp_mlp_init": (ctypes.c_int, [i32, ctypes.c_uint64, vp]),
+        "galp_mlp_predict": (ctypes.c_int, [vp, i32, vp, vp, i64, vp]),
+        "galp_mlp_loss_grad": (ctypes.c_int, [vp, i32, vp, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]),
+        "galp_mlp_train": (ctypes.c_int, [vp, i32, vp, vp, vp, i64, i32, i32, ctypes.c_float, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_float)]),
+        "galp_graph_set_mlp": (ctypes.c_int, [vp, i32, vp, ctypes.c_float]),
+        "galp_graph_log_enable": (ctypes.c_int, [vp, i64]),
+        "galp_graph_log_read": (ctypes.c_int, [vp, vp, vp, i64, ctypes.POINTER(i64)]),
+        "galp_graph_features": (ctypes.c_int, [vp, vp, vp, vp]),
+        "galp_graph_dispatch_stats": (ctypes.c_int, [vp, vp, i32, ctypes.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -301,3 +312,77 @@ class Graph:
 
     def profile_reset(self):
         self.ctx.check(lib().galp_profile_reset(self.h))
+
+
+    # ---- learned dispatch --------------------------------------------------
+    def set_mlp(self, hidden, params, cand_thresh):
+        p = np.ascontiguousarray(params, np.float32)
+        assert len(p) == mlp_num_params(hidden)
+        self.ctx.check(lib().galp_graph_set_mlp(self.h, hidden, p.ctypes.data, float(cand_thresh)))
+
+    def log_enable(self, capacity):
+        self.ctx.check(lib().galp_graph_log_enable(self.h, int(capacity)))
+
+    def log_read(self):
+        cnt = ctypes.c_int64()
+        self.ctx.check(lib().galp_graph_log_read(self.h, None, None, 0, ctypes.byref(cnt)))
+        k = cnt.value
+        x = np.empty((max(k, 1), 4), np.float32)
+        y = np.empty(max(k, 1), np.float32)
+        self.ctx.check(lib().galp_graph_log_read(self.h, x.ctypes.data, y.ctypes.data, k, ctypes.byref(cnt)))
+        return x[:k], y[:k]
+
+    def features(self):
+        d = np.empty(max(self.n, 1), np.float32)
+        mean = np.empty_like(d)
+        mx = np.empty_like(d)
+        self.ctx.check(lib().galp_graph_features(self.h, d.ctypes.data, mean.ctypes.data, mx.ctypes.data))
+        return d[:self.n], mean[:self.n], mx[:self.n]
+
+    def dispatch_stats(self):
+        out = np.zeros(32, np.int64)
+        cnt = ctypes.c_int32()
+        self.ctx.check(lib().galp_graph_dispatch_stats(self.h, out.ctypes.data, 32, ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+
+# ------------------------------------------------------------------- MLP ---
+def mlp_num_params(hidden):
+    return int(lib().galp_mlp_num_params(hidden))
+
+
+def mlp_init(hidden, seed):
+    p = np.empty(mlp_num_params(hidden), np.float32)
+    rc = lib().galp_mlp_init(hidden, seed, p.ctypes.data)
+    if rc != GALP_OK:
+        raise GalpError(rc, "galp_mlp_init")
+    return p
+
+
+def mlp_predict(ctx, hidden, params, x):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+    y = np.empty(max(len(x), 1), np.float32)
+    ctx.check(lib().galp_mlp_predict(ctx.h, hidden, p.ctypes.data, x.ctypes.data, len(x), y.ctypes.data))
+    return y[:len(x)]
+
+
+def mlp_loss_grad(ctx, hidden, params, x, y):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.empty(len(p), np.float32)
+    loss = ctypes.c_float()
+    ctx.check(lib().galp_mlp_loss_grad(ctx.h, hidden, p.ctypes.data, x.ctypes.data, y.ctypes.data, len(x),
+                                       g.ctypes.data, ctypes.byref(loss)))
+    return loss.value, g
+
+
+def mlp_train(ctx, hidden, params, x, y, epochs=20, batch=256, lr=1e-2, seed=1):
+    p = np.array(params, np.float32, copy=True)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+    y = np.ascontiguousarray(y, np.float32)
+    loss = ctypes.c_float()
+    ctx.check(lib().galp_mlp_train(ctx.h, hidden, p.ctypes.data, x.ctypes.data, y.ctypes.data, len(x), epochs,
+                                   batch, lr, seed, ctypes.byref(loss)))
+    return p, loss.value

@@ -97,12 +97,6 @@ static int bucket_of(int64_t d) {
     return NUM_BUCKETS - 1;
 }
 
-static int ceil_log2_host(int64_t x) {
-    int b = 0;
-    while (((int64_t)1 << b) < x) ++b;
-    return b;
-}
-
 template <class T>
 static int dev_alloc(galp_graph *g, T **p, int64_t count) {
     *p = nullptr;
@@ -124,6 +118,7 @@ static int dev_alloc(galp_graph *g, T **p, int64_t count) {
 }
 
 static void dev_free_all(galp_graph *g) {
+    learn_free(g);
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
                     g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
@@ -266,6 +261,20 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     }
     const std::vector<int64_t> &offi = g->reorder ? off_int : off;
 
+    // big vertices (classes W128..HUB) for the learned dispatcher, hubs first
+    g->big_vid.clear();
+    g->big_deg.clear();
+    for (int32_t x : hub_vertices) {
+        g->big_vid.push_back(x);
+        g->big_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
+    }
+    for (int b = B_W128; b < B_HUB; ++b)
+        for (int64_t k = 0; k < g->bucket[b].nv; ++k) {
+            const int32_t x = g->reorder ? (int32_t)(g->bucket[b].v0 + k) : vlist[(size_t)(g->bucket[b].v0 + k)];
+            g->big_vid.push_back(x);
+            g->big_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
+        }
+
     // hubs: phase-1 tiles and phase-2 (hub, label partition) items
     std::vector<int32_t> hub_nparts, item_hub, item_part, hub_tile0, hub_ntiles, tile_hub;
     std::vector<int64_t> tile_idx_off;
@@ -387,6 +396,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
     }
     GALP_HIP(ctx, hipStreamSynchronize(g->stream));
+    if (g->opts.dispatch == GALP_DISPATCH_LEARNED) GALP_TRY(learn_setup(g));
     g->preprocess_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return GALP_OK;
 }
@@ -394,7 +404,13 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
 static int launch_iteration(galp_graph *g) {
     switch (g->opts.method) {
     case GALP_METHOD_HASH:
+        if (g->opts.dispatch == GALP_DISPATCH_LEARNED) {
+            if (!g->learn || !g->learn->H)
+                return fail(g->ctx, GALP_ESTATE, "learned dispatch needs a network: call galp_graph_set_mlp");
+            GALP_TRY(launch_learned_front(g));
+        }
         GALP_TRY(launch_hash_iteration(g));
+        if (g->learn) GALP_TRY(learn_after_iteration(g));
         break;
     default:
         return fail(g->ctx, GALP_EUNSUPPORTED, "method " + std::to_string(g->opts.method) + " is not built yet");
@@ -533,6 +549,7 @@ GALP_API int galp_graph_query(galp_graph *g, galp_graph_info *out) {
 GALP_API int galp_lp_init(galp_graph *g) {
     if (!g) return GALP_EINVAL;
     GALP_TRY(dev_init_labels(g));
+    GALP_TRY(learn_reset_iteration(g));
     GALP_HIP(g->ctx, hipStreamSynchronize(g->stream));
     g->cur = 0;
     g->iter = 0;
@@ -546,6 +563,7 @@ GALP_API int galp_lp_set_labels(galp_graph *g, const int32_t *host_labels) {
         if (host_labels[v] < 0 || (int64_t)host_labels[v] >= g->n)
             return fail(g->ctx, GALP_EINVAL, "labels must lie in [0, n)");
     GALP_TRY(dev_init_labels(g));   // resets the control words
+    GALP_TRY(learn_reset_iteration(g));
     if (g->n) {
         int32_t *dst = g->d_lab[0];
         if (g->reorder) {
@@ -579,6 +597,10 @@ GALP_API int galp_lp_step(galp_graph *g, galp_step_stats *st) {
     const int idx = g->iter % kMaxIters;
     if (rc == GALP_OK) {
         hipError_t e = hipMemcpyAsync(g->h_pinned, g->d_iter_counts + idx, 8, hipMemcpyDeviceToHost, g->stream);
+        g->h_pinned[9] = 0;
+        if (e == hipSuccess && g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED)
+            e = hipMemcpyAsync(g->h_pinned + 9, g->learn->d_plist_n + kCtrOverflow, 4, hipMemcpyDeviceToHost,
+                               g->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(g->h_pinned + 2, g->d_stop, 4, hipMemcpyDeviceToHost, g->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
         if (e != hipSuccess) rc = fail(g->ctx, GALP_EHIP, std::string("step readback: ") + hipGetErrorString(e));
@@ -596,7 +618,7 @@ GALP_API int galp_lp_step(galp_graph *g, galp_step_stats *st) {
     g->iter += 1;
     if (st) {
         st->updated = cnt;
-        st->overflowed = 0;
+        st->overflowed = g->h_pinned[9];
         st->device_ms = ms;
         st->iteration = g->iter;
         st->stopped = g->h_pinned[2] != 0;
@@ -692,6 +714,19 @@ GALP_API int galp_profile_read(galp_graph *g, galp_kernel_stat *out, int32_t max
         out[i].total_ms = a.total_ms;
         out[i].alg_bytes = a.alg_bytes;
     }
+    return GALP_OK;
+}
+
+GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max, int32_t *count) {
+    if (!g || !count) return GALP_EINVAL;
+    *count = kNumCtrs;
+    if (!g->learn) {
+        for (int32_t i = 0; i < max && i < kNumCtrs; ++i) out[i] = 0;
+        return GALP_OK;
+    }
+    int64_t tmp[kNumCtrs];
+    GALP_HIP(g->ctx, hipMemcpy(tmp, g->learn->d_plist_n, sizeof(tmp), hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < max && i < kNumCtrs; ++i) out[i] = tmp[i];
     return GALP_OK;
 }
 

@@ -63,6 +63,25 @@ __device__ __forceinline__ T block_max(T x, T *scratch) {
     }
 }
 
+// Block-wide sum over NW waves; every thread gets the result.  Contains
+// barriers when NW > 1.
+template <int NW>
+__device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t *scratch) {
+    x = wave_sum_u32(x);
+    if constexpr (NW == 1) {
+        return x;
+    } else {
+        const int w = (int)(threadIdx.x / kWave);
+        if (lane_id() == 0) scratch[w] = x;
+        __syncthreads();
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) r += scratch[i];
+        __syncthreads();
+        return r;
+    }
+}
+
 // Adds a per-thread count into *counter with one atomic per block of NW waves.
 template <int NW>
 __device__ __forceinline__ void block_count_add(uint32_t mine, unsigned long long *counter,

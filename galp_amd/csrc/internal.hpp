@@ -29,6 +29,15 @@ enum BucketId {
     B_HUB,      // > 4096     label-partitioned blocks, ceil(deg/2048) per hub
     NUM_BUCKETS
 };
+// Upper degree bound of each class (host and device).
+__host__ __device__ constexpr int64_t bucket_hi(int b) {
+    return b < 5 ? ((int64_t)4 << b) : b < B_HUB ? ((int64_t)128 << (b - 5)) : INT64_MAX;
+}
+__host__ __device__ inline int bucket_of_deg(int64_t d) {
+    int b = 0;
+    while (b < B_HUB && d > bucket_hi(b)) ++b;
+    return b;
+}
 extern const int64_t kBucketHi[NUM_BUCKETS];
 extern const char *const kBucketName[NUM_BUCKETS];
 
@@ -65,6 +74,55 @@ struct Profiler {
     std::vector<hipEvent_t> pool;
     int find(const char *name);
 };
+
+// Candidate path of the learned dispatcher: a wave walks a tile of up to
+// kCandTile edges keeping up to kCandK labels in registers.
+constexpr int kCandTile = 1024;
+constexpr int kCandK = 8;
+constexpr int kCandCache = 32768;   // hottest labels held in LDS per workgroup
+
+// State of the learned dispatcher (allocated when dispatch is learned or
+// training samples are logged).  "Big" vertices are those of the classes
+// W128..HUB, the only ones the network routes; big vertices of degree >
+// kCandTile are "multi-tile" (mt): hubs first (mt index = hub index), then
+// the others.
+struct LearnState {
+    int H = 0;                         // hidden width of the attached MLP (0: none)
+    float *d_params = nullptr;
+    float *d_fdeg = nullptr, *d_fmean = nullptr, *d_fmax = nullptr;   // raw static features
+    int32_t *d_dist[2] = {nullptr, nullptr};   // distinct labels per vertex, double-buffered
+    int dcur = 0;                      // d_dist[dcur]: previous iteration's counts
+    int64_t big_v0 = 0, big_n = 0;     // big vertex i: vlist ? vlist[big_v0+i] : big_v0+i
+    float cand_thresh = 4.0f;          // predicted distinct labels routed to candidates
+    // candidate work list (tiles)
+    int64_t cw_cap = 0;
+    int32_t *d_cw_v = nullptr, *d_cw_len = nullptr, *d_cw_pslot = nullptr;
+    int64_t *d_cw_e0 = nullptr;
+    // multi-tile vertices and their partial candidates
+    int64_t nmt = 0, nctiles = 0;
+    int32_t *d_mt_vid = nullptr, *d_mt_of = nullptr;        // mt -> vertex, vertex -> mt or -1
+    int32_t *d_mt_ctile0 = nullptr, *d_mt_nctiles = nullptr;
+    int32_t *d_mt_cand = nullptr;      // mt vertex routed to candidates this iteration
+    int32_t *d_pn = nullptr;           // candidates per tile, -1 on overflow
+    uint32_t *d_pkey = nullptr, *d_pcnt = nullptr, *d_plast = nullptr;
+    int32_t *d_hub_sel = nullptr;      // hub left to the plain hub path this iteration
+    uint32_t *d_hub_distinct = nullptr;
+    // plain class lists of this iteration (classes W128..B4K)
+    int32_t *plist[NUM_BUCKETS] = {};
+    int32_t **d_plist = nullptr;       // device copy of plist
+    // device counters: [0..NUM_BUCKETS) list sizes, then candidate tiles,
+    // candidate overflows, vertices routed to candidates
+    int64_t *d_plist_n = nullptr;
+    // training-sample log: x[cap][4], y[cap]
+    int64_t log_cap = 0;
+    float *d_log_x = nullptr, *d_log_y = nullptr;
+    int64_t *d_log_n = nullptr;
+    int32_t *distinct_out(const galp_graph *) const { return d_dist[dcur ^ 1]; }
+};
+constexpr int kCtrCandTiles = NUM_BUCKETS;
+constexpr int kCtrOverflow = NUM_BUCKETS + 1;
+constexpr int kCtrCandVerts = NUM_BUCKETS + 2;
+constexpr int kNumCtrs = NUM_BUCKETS + 3;
 
 }  // namespace galp
 
@@ -121,6 +179,9 @@ struct galp_graph {
     int64_t device_bytes = 0;
 
     galp::Profiler prof;
+    galp::LearnState *learn = nullptr;
+    std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
+    std::vector<int64_t> big_deg;
 };
 
 namespace galp {
@@ -133,6 +194,13 @@ int fail(galp_ctx *ctx, int code, const std::string &msg);
 int launch_hash_iteration(galp_graph *g);
 // Iteration bookkeeping: records the change count, sets the stop flag.
 int launch_iter_end(galp_graph *g);
+
+// Learned dispatch (learned.hip).
+int learn_setup(galp_graph *g);
+void learn_free(galp_graph *g);
+int learn_reset_iteration(galp_graph *g);
+int launch_learned_front(galp_graph *g);
+int learn_after_iteration(galp_graph *g);
 
 // Graph bookkeeping kernels (graph_dev.hip), all on g->stream.
 int dev_init_labels(galp_graph *g);

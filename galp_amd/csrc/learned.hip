@@ -1,0 +1,793 @@
+// learned.hip -- the learned dispatcher of the hash method.
+//
+// Each iteration, a dispatch kernel runs the MLP (mlp.hpp) on every vertex
+// of degree > 64 ("big" vertices) and routes it:
+//   - predicted distinct labels <= cand_thresh: the candidate path -- one
+//     wave per tile of <= kCandTile edges keeps up to kCandK (label, count,
+//     last position) candidates in registers and matches each 64-edge chunk
+//     against them with ballots: no LDS table, so the workgroup's LDS holds
+//     the kCandCache hottest labels (the lowest ids after the degree-
+//     descending renumbering) and gathers that hit them skip L2;
+//   - otherwise: the plain degree-class kernels of lp_hash.hip, through
+//     per-class vertex lists built here.
+// A tile that meets more than kCandK distinct labels sends its vertex to the
+// plain class list in the same iteration (hub tiles: the hub to the plain
+// hub path), so a misprediction costs time, never labels.  Every path
+// records the number of distinct labels it saw, the network's fourth input
+// for the next iteration.
+// No reference counterpart: the reference picks a kernel by -l only
+// (lfht.cuh:127-165).
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+#include "mlp.hpp"
+#include "pipeline.hpp"
+
+namespace galp {
+
+// log2-scaled network inputs from the raw features and the previous count
+__device__ __forceinline__ void mlp_inputs(float deg, float mean, float mx, int32_t prevd, float *x) {
+    x[0] = __log2f(1.f + deg);
+    x[1] = __log2f(1.f + mean);
+    x[2] = __log2f(1.f + mx);
+    x[3] = prevd >= 0 ? __log2f(1.f + (float)prevd) : -1.f;
+}
+
+
+// ---------------------------------------------------------------------------
+// Static features: degree, mean and max neighbour degree (one wave per big
+// vertex, once at upload).  mean = (float)((double)sum / deg), the exact
+// integer sum rounded once.
+template <typename E>
+__global__ void __launch_bounds__(256)
+k_features(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ vlist,
+           int64_t v0, int64_t nv, float *__restrict__ fdeg, float *__restrict__ fmean, float *__restrict__ fmax) {
+    const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
+    const int64_t nw = (int64_t)gridDim.x * (256 / kWave);
+    for (int64_t i = wave; i < nv; i += nw) {
+        const int64_t v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
+        const int64_t b = (int64_t)off[v], d = (int64_t)off[v + 1] - b;
+        unsigned long long sum = 0, mx = 0;
+        for (int64_t j = lane_id(); j < d; j += kWave) {
+            const int64_t u = nbr[b + j];
+            const unsigned long long du = (unsigned long long)((int64_t)off[u + 1] - (int64_t)off[u]);
+            sum += du;
+            mx = du > mx ? du : mx;
+        }
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) {
+            sum += __shfl_xor(sum, s);
+            const unsigned long long o = __shfl_xor(mx, s);
+            mx = o > mx ? o : mx;
+        }
+        if (lane_id() == 0) {
+            fdeg[v] = (float)d;
+            fmean[v] = d ? (float)((double)sum / (double)d) : 0.f;
+            fmax[v] = (float)mx;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch.  Wave-aggregated appends: one atomic per wave and list.
+__device__ __forceinline__ int64_t wave_append(int64_t *ctr, bool want) {
+    const unsigned long long ball = __ballot(want);
+    if (!ball) return -1;
+    const int leader = __ffsll((long long)ball) - 1;
+    unsigned long long base = 0;
+    if (lane_id() == leader) base = atomicAdd((unsigned long long *)ctr, (unsigned long long)__popcll(ball));
+    base = __shfl(base, leader);
+    return want ? (int64_t)base + __popcll(ball & ((1ull << lane_id()) - 1ull)) : -1;
+}
+
+__global__ void k_reset_ctrs(int64_t *ctr) {
+    if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
+}
+
+template <int H, typename E>
+__global__ void __launch_bounds__(256)
+k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t big_v0, int64_t big_n,
+           const float *__restrict__ params, const float *__restrict__ fdeg, const float *__restrict__ fmean,
+           const float *__restrict__ fmax, const int32_t *__restrict__ dprev, float cand_thresh, int64_t nhubs,
+           const int32_t *__restrict__ mt_of, const int32_t *__restrict__ mt_ctile0,
+           int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel, int32_t *__restrict__ cw_v,
+           int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len, int32_t *__restrict__ cw_pslot,
+           int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr, const int32_t *__restrict__ stop) {
+    __shared__ float sp[mlp_num_params(H)];
+    if (*stop) return;
+    for (int i = threadIdx.x; i < mlp_num_params(H); i += 256) sp[i] = params[i];
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < big_n; base += (int64_t)gridDim.x * 256) {
+        const int64_t i = base + threadIdx.x;
+        const bool ok = i < big_n;
+        int64_t v = 0, b = 0, d = 0;
+        bool cand = false;
+        int cls = -1, mt = -1;
+        if (ok) {
+            v = vlist ? (int64_t)vlist[big_v0 + i] : big_v0 + i;
+            b = (int64_t)off[v];
+            d = (int64_t)off[v + 1] - b;
+            float x[4];
+            mlp_inputs(fdeg[v], fmean[v], fmax[v], dprev[v], x);
+            const float y = mlp_forward<H>(sp, x[0], x[1], x[2], x[3]);
+            cand = exp2f(y) - 1.f <= cand_thresh;
+            cls = bucket_of_deg(d);
+            mt = d > kCandTile ? mt_of[v] : -1;
+        }
+        // candidates of one tile
+        const int64_t pos = wave_append(ctr + kCtrCandTiles, ok && cand && mt < 0);
+        if (pos >= 0) {
+            cw_v[pos] = (int32_t)v;
+            cw_e0[pos] = b;
+            cw_len[pos] = (int32_t)d;
+            cw_pslot[pos] = -1;
+        }
+        (void)wave_append(ctr + kCtrCandVerts, ok && cand);
+        // multi-tile vertices: ceil(deg / kCandTile) tiles with partial slots
+        if (mt >= 0) {
+            mt_cand[mt] = cand ? 1 : 0;
+            if (mt < nhubs) hub_sel[mt] = cand ? 0 : 1;
+            if (cand) {
+                const int nt = (int)((d + kCandTile - 1) / kCandTile);
+                const int64_t p0 = (int64_t)atomicAdd((unsigned long long *)(ctr + kCtrCandTiles),
+                                                      (unsigned long long)nt);
+                for (int t = 0; t < nt; ++t) {
+                    cw_v[p0 + t] = (int32_t)v;
+                    cw_e0[p0 + t] = b + (int64_t)t * kCandTile;
+                    cw_len[p0 + t] = (int32_t)min((int64_t)kCandTile, d - (int64_t)t * kCandTile);
+                    cw_pslot[p0 + t] = mt_ctile0[mt] + t;
+                }
+            }
+        }
+        // plain classes W128..B4K
+#pragma unroll
+        for (int c = B_W128; c < B_HUB; ++c) {
+            const int64_t q = wave_append(ctr + c, ok && !cand && cls == c);
+            if (q >= 0) plist[c][q] = (int32_t)v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate path.  One wave per tile (pipelined over the wave's tiles);
+// candidates live in wave-uniform registers.
+template <int EPT, typename E>
+struct CandPolicy {
+    struct Item {
+        bool valid;
+        int len;
+        int64_t e0;
+        int32_t v;
+        int32_t pslot;
+        uint32_t row0;    // position of the tile's first edge in the row
+        int32_t old;
+    };
+    const E *off;
+    const int32_t *prev;
+    int32_t *next;
+    const int32_t *cw_v, *cw_len, *cw_pslot;
+    const int64_t *cw_e0;
+    int64_t ntiles;
+    uint32_t *pkey, *pcnt, *plast;
+    int32_t *pn;
+    int32_t *dist_out;
+    int32_t *const *plist;
+    int64_t *ctr;
+    const int32_t *cache;   // LDS copy of labels [0, hot)
+    int32_t hot;
+    uint32_t changed;
+
+    __device__ Item item(int64_t t) const {
+        Item it{false, 0, 0, 0, -1, 0, 0};
+        if (t < ntiles) {
+            it.v = cw_v[t];
+            it.e0 = cw_e0[t];
+            it.len = cw_len[t];
+            it.pslot = cw_pslot[t];
+            it.row0 = (uint32_t)(it.e0 - (int64_t)off[it.v]);
+            it.old = prev[it.v];
+            it.valid = true;
+        }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.e0; }
+
+    // lab holds prev[u] for cold neighbours; hot ones are re-read from LDS
+    // (the driver's gather still issued a load, to prev[0] for them, which
+    // all such lanes share, so it costs one line instead of one per lane)
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT]) {
+        const int lane = lane_id();
+        int ncand = 0;
+        int32_t ck[kCandK];
+        uint32_t cc[kCandK], cl[kCandK];
+#pragma unroll
+        for (int j = 0; j < kCandK; ++j) {
+            ck[j] = -1;
+            cc[j] = 0;
+            cl[j] = 0;
+        }
+        bool overflow = false;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * kWave + lane;
+            if (k * kWave >= it.len || overflow) break;   // uniform
+            const bool ok = pos < it.len;
+            const int32_t u = ids[k];
+            const int32_t L = ok ? (u < hot ? cache[u] : lab[k]) : -1;
+            const uint32_t rowpos = it.row0 + (uint32_t)(k * kWave);
+            bool matched = false;
+#pragma unroll
+            for (int j = 0; j < kCandK; ++j) {
+                if (j < ncand) {
+                    const bool eq = L == ck[j];
+                    const unsigned long long m = __ballot(eq);
+                    if (m) {
+                        cc[j] += (uint32_t)__popcll(m);
+                        cl[j] = rowpos + (uint32_t)(63 - __clzll((long long)m));
+                    }
+                    matched = matched || eq;
+                }
+            }
+            unsigned long long rest = __ballot(ok && !matched);
+            while (rest) {
+                if (ncand == kCandK) {
+                    overflow = true;
+                    break;
+                }
+                const int32_t c = __builtin_amdgcn_readlane(L, __ffsll((long long)rest) - 1);
+                const unsigned long long m = __ballot(ok && L == c);
+#pragma unroll
+                for (int j = 0; j < kCandK; ++j)
+                    if (j == ncand) {
+                        ck[j] = c;
+                        cc[j] = (uint32_t)__popcll(m);
+                        cl[j] = rowpos + (uint32_t)(63 - __clzll((long long)m));
+                    }
+                ++ncand;
+                rest &= ~m;
+            }
+        }
+        if (lane != 0) return;
+        if (it.pslot >= 0) {   // hub tile: partial candidates
+            pn[it.pslot] = overflow ? -1 : ncand;
+            if (!overflow)
+#pragma unroll
+                for (int j = 0; j < kCandK; ++j)
+                    if (j < ncand) {
+                        pkey[(int64_t)it.pslot * kCandK + j] = (uint32_t)ck[j];
+                        pcnt[(int64_t)it.pslot * kCandK + j] = cc[j];
+                        plast[(int64_t)it.pslot * kCandK + j] = cl[j];
+                    }
+            return;
+        }
+        if (overflow) {   // the plain class kernel of this degree takes the vertex
+            const int c = bucket_of_deg(it.len);
+            const unsigned long long q = atomicAdd((unsigned long long *)(ctr + c), 1ull);
+            plist[c][q] = it.v;
+            atomicAdd((unsigned long long *)(ctr + kCtrOverflow), 1ull);
+            return;
+        }
+        unsigned long long best = 0;
+        int32_t w = 0;
+#pragma unroll
+        for (int j = 0; j < kCandK; ++j)
+            if (j < ncand) {
+                const unsigned long long pk = ((unsigned long long)cc[j] << 32) | cl[j];
+                if (pk > best) {
+                    best = pk;
+                    w = ck[j];
+                }
+            }
+        next[it.v] = w;
+        changed += (w != it.old);
+        if (dist_out) dist_out[it.v] = ncand;
+    }
+};
+
+// The candidate path's own pipeline: like pipelined(), plus the neighbour
+// ids of each item are kept for the LDS label cache.
+template <int EPT, class Policy>
+__device__ __forceinline__ void cand_pipeline(Policy &pol, const int32_t *__restrict__ nbr,
+                                              const int32_t *__restrict__ prev, int64_t first, int64_t stride,
+                                              int32_t hot) {
+    using Item = typename Policy::Item;
+    const int lane = lane_id();
+    int64_t i = first;
+    auto load_nbr = [&](int32_t (&u)[EPT], const Item &it) {
+        const int64_t e0 = it.valid ? it.e0 : 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * kWave + lane;
+            const bool ok = it.valid && pos < it.len;
+            u[k] = __builtin_nontemporal_load(&nbr[ok ? e0 + pos : 0]);
+        }
+    };
+    auto gather = [&](int32_t (&lab)[EPT], int32_t (&ids)[EPT], const int32_t (&u)[EPT], const Item &it) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * kWave + lane;
+            const bool ok = it.valid && pos < it.len;
+            ids[k] = ok ? u[k] : 0;
+            lab[k] = prev[(ok && u[k] >= hot) ? u[k] : 0];
+        }
+    };
+    Item s0 = pol.item(i), s1 = pol.item(i + stride), s2 = pol.item(i + 2 * stride);
+    int32_t u[EPT], labA[EPT], labB[EPT], idA[EPT], idB[EPT];
+    load_nbr(u, s0);
+    gather(labA, idA, u, s0);
+    load_nbr(u, s1);
+    while (s0.valid) {
+        gather(labB, idB, u, s1);
+        load_nbr(u, s2);
+        Item s3 = pol.item(i + 3 * stride);
+        pol.process(s0, labA, idA);
+        s0 = s1; s1 = s2; s2 = s3; i += stride;
+        if (!s0.valid) break;
+        gather(labA, idA, u, s1);
+        load_nbr(u, s2);
+        s3 = pol.item(i + 3 * stride);
+        pol.process(s0, labB, idB);
+        s0 = s1; s1 = s2; s2 = s3; i += stride;
+    }
+}
+
+template <typename E>
+__global__ void __launch_bounds__(1024)
+k_cand(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+       int32_t *__restrict__ next, const int32_t *__restrict__ cw_v, const int64_t *__restrict__ cw_e0,
+       const int32_t *__restrict__ cw_len, const int32_t *__restrict__ cw_pslot, uint32_t *__restrict__ pkey,
+       uint32_t *__restrict__ pcnt, uint32_t *__restrict__ plast, int32_t *__restrict__ pn,
+       int32_t *__restrict__ dist_out, int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr,
+       int32_t hot, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
+    constexpr int EPT = kCandTile / kWave;
+    __shared__ int32_t cache[kCandCache];
+    if (*stop) return;
+    for (int i = threadIdx.x; i < hot; i += 1024) cache[i] = prev[i];
+    __syncthreads();
+    const int64_t ntiles = ctr[kCtrCandTiles];
+    const int64_t wave = (int64_t)blockIdx.x * 16 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int64_t nwaves = (int64_t)gridDim.x * 16;
+    CandPolicy<EPT, E> pol{off, prev,  next,  cw_v,     cw_len, cw_pslot, cw_e0, ntiles, pkey, pcnt,
+                           plast, pn,  dist_out, plist, ctr,    cache,    hot,   0u};
+    cand_pipeline<EPT>(pol, nbr, prev, wave, nwaves, hot);
+    const uint32_t s = wave_sum_u32(pol.changed);
+    if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
+}
+
+// Multi-tile vertices on the candidate path: merge the tiles' partial
+// candidates (one block per vertex, LDS table).  An overflowed tile, or more
+// distinct labels than the table holds, hands the vertex to the plain path
+// of its class (hubs: the plain hub path), which runs after this kernel.
+template <typename E>
+__global__ void __launch_bounds__(256)
+k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int64_t nmt, int64_t nhubs,
+             const int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel,
+             const int32_t *__restrict__ mt_ctile0, const int32_t *__restrict__ mt_nctiles,
+             int32_t *const *__restrict__ plist,
+                 const uint32_t *__restrict__ pkey, const uint32_t *__restrict__ pcnt,
+                 const uint32_t *__restrict__ plast, const int32_t *__restrict__ pn,
+                 const int32_t *__restrict__ prev, int32_t *__restrict__ next, int32_t *__restrict__ dist_out,
+                 unsigned long long *__restrict__ counter, int64_t *__restrict__ ctr,
+                 const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+    constexpr int BITS = 11, CAP = 1 << BITS;
+    __shared__ uint32_t skey[CAP], scnt[CAP], slast[CAP], slist[CAP];
+    __shared__ uint32_t nlist;
+    __shared__ int32_t bad;
+    __shared__ unsigned long long sred64[4];
+    if (*stop) return;
+    uint32_t changed = 0;
+    for (int s = threadIdx.x; s < CAP; s += 256) {
+        skey[s] = kEmpty;
+        scnt[s] = 0;
+        slast[s] = 0;
+    }
+    for (int64_t h = blockIdx.x; h < nmt; h += gridDim.x) {
+        if (!mt_cand[h]) continue;   // uniform per block
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            nlist = 0;
+            bad = 0;
+        }
+        __syncthreads();
+        const int64_t t0 = mt_ctile0[h], nt = mt_nctiles[h];
+        for (int64_t t = t0 + threadIdx.x; t < t0 + nt; t += 256)
+            if (pn[t] < 0) bad = 1;
+        __syncthreads();
+        if (!bad) {
+            for (int64_t e = t0 * kCandK + threadIdx.x; e < (t0 + nt) * kCandK; e += 256) {
+                const int64_t t = e / kCandK;
+                if ((int)(e % kCandK) >= pn[t]) continue;
+                const uint32_t L = pkey[e];
+                uint32_t hs = hash_slot(L, BITS);
+                bool placed = false, created = false;
+                for (int probe = 0; probe < CAP; ++probe) {
+                    const uint32_t cur = skey[hs];
+                    if (cur == L) { placed = true; break; }
+                    if (cur == kEmpty) {
+                        const uint32_t o = atomicCAS(&skey[hs], kEmpty, L);
+                        if (o == kEmpty) { placed = created = true; break; }
+                        if (o == L) { placed = true; break; }
+                    }
+                    hs = (hs + 1) & (CAP - 1);
+                }
+                if (!placed) {   // more distinct labels than slots: plain path
+                    bad = 1;
+                    continue;
+                }
+                atomicAdd(&scnt[hs], pcnt[e]);
+                atomicMax(&slast[hs], plast[e]);
+                if (created) slist[atomicAdd(&nlist, 1u)] = hs;
+            }
+        }
+        __syncthreads();
+        const uint32_t nl = nlist;
+        unsigned long long best = 0;
+        uint32_t bestkey = 0;
+        for (uint32_t i = threadIdx.x; i < nl; i += 256) {
+            const uint32_t sl = slist[i];
+            const unsigned long long pk = ((unsigned long long)scnt[sl] << 32) | slast[sl];
+            if (pk > best) {
+                best = pk;
+                bestkey = skey[sl];
+            }
+        }
+        const unsigned long long top = block_max<4>(best, sred64);
+        __shared__ uint32_t swin;
+        if (best == top && top) swin = bestkey;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nl; i += 256) {
+            const uint32_t sl = slist[i];
+            skey[sl] = kEmpty;
+            scnt[sl] = 0;
+            slast[sl] = 0;
+        }
+        if (threadIdx.x == 0) {
+            const int64_t v = mt_vid[h];
+            if (bad) {
+                if (h < nhubs) {
+                    hub_sel[h] = 1;
+                } else {
+                    const int c = bucket_of_deg((int64_t)off[v + 1] - (int64_t)off[v]);
+                    const unsigned long long q = atomicAdd((unsigned long long *)(ctr + c), 1ull);
+                    plist[c][q] = (int32_t)v;
+                }
+                atomicAdd((unsigned long long *)(ctr + kCtrOverflow), 1ull);
+            } else {
+                const int32_t w = (int32_t)swin;
+                next[v] = w;
+                changed += (w != prev[v]);
+                if (dist_out) dist_out[v] = (int32_t)nl;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && changed) atomicAdd(counter, (unsigned long long)changed);
+    (void)err;
+}
+
+// Training-sample log: for every big vertex, (inputs from the previous
+// count, log2(1 + this iteration's count)).
+template <typename E>
+__global__ void __launch_bounds__(256)
+k_log_samples(const int32_t *__restrict__ vlist, int64_t big_v0, int64_t big_n, const float *__restrict__ fdeg,
+              const float *__restrict__ fmean, const float *__restrict__ fmax, const int32_t *__restrict__ dprev,
+              const int32_t *__restrict__ dcur, float *__restrict__ lx, float *__restrict__ ly, int64_t cap,
+              int64_t *__restrict__ ln, const int32_t *__restrict__ stop) {
+    if (*stop) return;
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < big_n; base += (int64_t)gridDim.x * 256) {
+        const int64_t i = base + threadIdx.x;
+        const bool ok = i < big_n;
+        const int64_t pos = wave_append(ln, ok);
+        if (ok && pos < cap) {
+            const int64_t v = vlist ? (int64_t)vlist[big_v0 + i] : big_v0 + i;
+            float x[4];
+            mlp_inputs(fdeg[v], fmean[v], fmax[v], dprev[v], x);
+            for (int k = 0; k < 4; ++k) lx[4 * pos + k] = x[k];
+            ly[pos] = __log2f(1.f + (float)dcur[v]);
+        }
+    }
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t val) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+__global__ void k_gather_feat(const float *__restrict__ f, const int32_t *__restrict__ perm, int64_t n,
+                              float *__restrict__ out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        out[v] = f[perm ? perm[v] : v];
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static int grid_cap(const galp_graph *g, int64_t work, int per_cu) {
+    const int64_t cap = (int64_t)g->ctx->prop.multiProcessorCount * per_cu;
+    return (int)std::max<int64_t>(1, std::min(work, cap));
+}
+
+template <class T>
+static int lalloc(galp_graph *g, T **p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(g->ctx, GALP_ENOMEM, "learned-dispatch allocation failed");
+    }
+    g->device_bytes += (int64_t)(sizeof(T) * (size_t)count);
+    return GALP_OK;
+}
+
+#define GALP_LTRY(expr)                 \
+    do {                                \
+        int _rc = (expr);               \
+        if (_rc != GALP_OK) return _rc; \
+    } while (0)
+
+void learn_free(galp_graph *g) {
+    LearnState *L = g->learn;
+    if (!L) return;
+    void *ptrs[] = {L->d_params,  L->d_fdeg,       L->d_fmean,    L->d_fmax,    L->d_dist[0],  L->d_dist[1],
+                    L->d_cw_v,    L->d_cw_len,     L->d_cw_pslot, L->d_cw_e0,   L->d_mt_vid,   L->d_mt_of,
+                    L->d_mt_ctile0, L->d_mt_nctiles, L->d_mt_cand, L->d_pn,     L->d_pkey,     L->d_pcnt,
+                    L->d_plast,   L->d_hub_sel,    L->d_hub_distinct, L->d_plist, L->d_plist_n, L->d_log_x,
+                    L->d_log_y,   L->d_log_n};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int b = 0; b < NUM_BUCKETS; ++b)
+        if (L->plist[b]) (void)hipFree(L->plist[b]);
+    delete L;
+    g->learn = nullptr;
+}
+
+// Allocates the learned-dispatch state and computes the static features.
+int learn_setup(galp_graph *g) {
+    if (g->learn) return GALP_OK;
+    LearnState *L = new (std::nothrow) LearnState();
+    if (!L) return fail(g->ctx, GALP_ENOMEM, "host allocation failed");
+    g->learn = L;
+    const int64_t n = g->n;
+    // big vertices: one contiguous range of ids (degree-descending
+    // renumbering: hubs first) or of the class-ordered vertex lists
+    int64_t big_n = 0;
+    for (int b = B_W128; b <= B_HUB; ++b) big_n += g->bucket[b].nv;
+    L->big_n = big_n;
+    L->big_v0 = g->reorder ? 0 : g->bucket[B_W128].v0;
+    GALP_LTRY(lalloc(g, &L->d_fdeg, n));
+    GALP_LTRY(lalloc(g, &L->d_fmean, n));
+    GALP_LTRY(lalloc(g, &L->d_fmax, n));
+    GALP_LTRY(lalloc(g, &L->d_dist[0], n));
+    GALP_LTRY(lalloc(g, &L->d_dist[1], n));
+    GALP_LTRY(lalloc(g, &L->d_plist_n, kNumCtrs));
+    GALP_LTRY(lalloc(g, &L->d_log_n, 1));
+    for (int b = B_W128; b < B_HUB; ++b) GALP_LTRY(lalloc(g, &L->plist[b], g->bucket[b].nv));
+    GALP_LTRY(lalloc(g, &L->d_plist, NUM_BUCKETS));
+    if (hipMemcpy(L->d_plist, L->plist, sizeof(L->plist), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "class list table upload failed");
+    // multi-tile vertices: hubs first (g->big_vid lists hubs first)
+    std::vector<int32_t> mt_vid, ctile0, nctiles, mt_of((size_t)n, -1);
+    int64_t nct = 0;
+    for (size_t k = 0; k < g->big_vid.size(); ++k) {
+        const int64_t d = g->big_deg[k];
+        if (d <= kCandTile) continue;
+        mt_of[(size_t)g->big_vid[k]] = (int32_t)mt_vid.size();
+        mt_vid.push_back(g->big_vid[k]);
+        const int32_t nt = (int32_t)((d + kCandTile - 1) / kCandTile);
+        ctile0.push_back((int32_t)nct);
+        nctiles.push_back(nt);
+        nct += nt;
+    }
+    for (int64_t h = 0; h < g->nhubs; ++h)   // hub h is multi-tile vertex h
+        if (mt_of[(size_t)g->big_vid[(size_t)h]] != (int32_t)h)
+            return fail(g->ctx, GALP_EINVAL, "internal: hub order");
+    L->nmt = (int64_t)mt_vid.size();
+    L->nctiles = nct;
+    L->cw_cap = (big_n - L->nmt) + nct;
+    GALP_LTRY(lalloc(g, &L->d_cw_v, L->cw_cap));
+    GALP_LTRY(lalloc(g, &L->d_cw_len, L->cw_cap));
+    GALP_LTRY(lalloc(g, &L->d_cw_pslot, L->cw_cap));
+    GALP_LTRY(lalloc(g, &L->d_cw_e0, L->cw_cap));
+    GALP_LTRY(lalloc(g, &L->d_mt_vid, L->nmt));
+    GALP_LTRY(lalloc(g, &L->d_mt_of, n));
+    GALP_LTRY(lalloc(g, &L->d_mt_ctile0, L->nmt));
+    GALP_LTRY(lalloc(g, &L->d_mt_nctiles, L->nmt));
+    GALP_LTRY(lalloc(g, &L->d_mt_cand, L->nmt));
+    GALP_LTRY(lalloc(g, &L->d_pn, nct));
+    GALP_LTRY(lalloc(g, &L->d_pkey, nct * kCandK));
+    GALP_LTRY(lalloc(g, &L->d_pcnt, nct * kCandK));
+    GALP_LTRY(lalloc(g, &L->d_plast, nct * kCandK));
+    GALP_LTRY(lalloc(g, &L->d_hub_sel, g->nhubs));
+    GALP_LTRY(lalloc(g, &L->d_hub_distinct, g->nhubs));
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(L->d_mt_vid, mt_vid.data(), 4 * mt_vid.size()) || !up(L->d_mt_ctile0, ctile0.data(), 4 * ctile0.size()) ||
+        !up(L->d_mt_nctiles, nctiles.data(), 4 * nctiles.size()) || !up(L->d_mt_of, mt_of.data(), 4 * mt_of.size()))
+        return fail(g->ctx, GALP_EHIP, "multi-tile tables upload failed");
+    const int gd = grid_cap(g, (n + 255) / 256, 32);
+    k_fill_i32<<<gd, 256, 0, g->stream>>>(L->d_dist[0], n, -1);
+    k_fill_i32<<<gd, 256, 0, g->stream>>>(L->d_dist[1], n, -1);
+    k_fill_i32<<<gd, 256, 0, g->stream>>>((int32_t *)L->d_fdeg, n, 0);   // 0.0f: features of small vertices
+    k_fill_i32<<<gd, 256, 0, g->stream>>>((int32_t *)L->d_fmean, n, 0);
+    k_fill_i32<<<gd, 256, 0, g->stream>>>((int32_t *)L->d_fmax, n, 0);
+    k_fill_i32<<<grid_cap(g, (L->nmt + 255) / 256, 8), 256, 0, g->stream>>>(L->d_mt_cand, L->nmt, 0);
+    k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_sel, g->nhubs, 1);
+    k_fill_i32<<<1, 256, 0, g->stream>>>((int32_t *)L->d_hub_distinct, g->nhubs, 0);
+    k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_log_n);
+    if (big_n) {
+        const int gf = grid_cap(g, (big_n + 3) / 4, 64);
+        if (g->off64)
+            k_features<int64_t><<<gf, 256, 0, g->stream>>>((const int64_t *)g->d_off, g->d_nbr, g->d_vlist,
+                                                           L->big_v0, big_n, L->d_fdeg, L->d_fmean, L->d_fmax);
+        else
+            k_features<int32_t><<<gf, 256, 0, g->stream>>>((const int32_t *)g->d_off, g->d_nbr, g->d_vlist,
+                                                           L->big_v0, big_n, L->d_fdeg, L->d_fmean, L->d_fmax);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "feature kernels failed");
+    return GALP_OK;
+}
+
+int learn_reset_iteration(galp_graph *g) {
+    LearnState *L = g->learn;
+    if (L) L->dcur = 0;
+    if (L) {
+        const int gd = grid_cap(g, (g->n + 255) / 256, 32);
+        k_fill_i32<<<gd, 256, 0, g->stream>>>(L->d_dist[0], g->n, -1);
+        k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_sel, g->nhubs, 1);
+        k_fill_i32<<<grid_cap(g, (L->nmt + 255) / 256, 8), 256, 0, g->stream>>>(L->d_mt_cand, L->nmt, 0);
+        k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_plist_n);
+    }
+    return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "reset kernels failed");
+}
+
+template <int H, typename E>
+static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl) {
+    k_dispatch<H, E><<<grid_cap(g, (L->big_n + 255) / 256, 8), 256, 0, g->stream>>>(
+        (const E *)g->d_off, g->d_vlist, L->big_v0, L->big_n, L->d_params, L->d_fdeg, L->d_fmean, L->d_fmax,
+        L->d_dist[L->dcur], L->cand_thresh, g->nhubs, L->d_mt_of, L->d_mt_ctile0, L->d_mt_cand, L->d_hub_sel, L->d_cw_v,
+        L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, dpl, L->d_plist_n, g->d_stop);
+}
+
+template <typename E>
+static int learned_front(galp_graph *g) {
+    LearnState *L = g->learn;
+    int32_t *const *dpl = L->d_plist;
+    const int32_t *prev = g->d_lab[g->cur];
+    int32_t *next = g->d_lab[g->cur ^ 1];
+    PendingLaunch pl;
+    k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_plist_n);
+    prof_begin(g, "dispatch", 0.0, &pl);
+    switch (L->H) {
+    case 8: launch_dispatch<8, E>(g, L, dpl); break;
+    case 16: launch_dispatch<16, E>(g, L, dpl); break;
+    default: launch_dispatch<32, E>(g, L, dpl); break;
+    }
+    prof_end(g, &pl);
+    double cand_bytes = 0.0;   // edges of the candidate path are not known on the host
+    prof_begin(g, "cand", cand_bytes, &pl);
+    const int32_t hot = g->reorder ? (int32_t)std::min<int64_t>(kCandCache, g->n) : 0;
+    k_cand<E><<<g->ctx->prop.multiProcessorCount, 1024, 0, g->stream>>>(
+        (const E *)g->d_off, g->d_nbr, prev, next, L->d_cw_v, L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, L->d_pkey,
+        L->d_pcnt, L->d_plast, L->d_pn, L->distinct_out(g), dpl, L->d_plist_n, hot, g->d_counter, g->d_stop);
+    prof_end(g, &pl);
+    if (L->nmt) {
+        prof_begin(g, "cand_merge", 0.0, &pl);
+        k_cand_merge<E><<<grid_cap(g, L->nmt, 4), 256, 0, g->stream>>>(
+            (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
+            L->d_mt_nctiles, dpl, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, prev, next, L->distinct_out(g),
+            g->d_counter, L->d_plist_n, g->d_stop, g->d_err);
+        prof_end(g, &pl);
+    }
+    return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "learned dispatch launch failed");
+}
+
+int launch_learned_front(galp_graph *g) {
+    return g->off64 ? learned_front<int64_t>(g) : learned_front<int32_t>(g);
+}
+
+// After the iteration's LP kernels: log samples (if enabled), swap counts.
+int learn_after_iteration(galp_graph *g) {
+    LearnState *L = g->learn;
+    if (!L) return GALP_OK;
+    if (L->log_cap && L->big_n) {
+        k_log_samples<int32_t><<<grid_cap(g, (L->big_n + 255) / 256, 8), 256, 0, g->stream>>>(
+            g->d_vlist, L->big_v0, L->big_n, L->d_fdeg, L->d_fmean, L->d_fmax, L->d_dist[L->dcur],
+            L->d_dist[L->dcur ^ 1], L->d_log_x, L->d_log_y, L->log_cap, L->d_log_n, g->d_stop);
+    }
+    L->dcur ^= 1;
+    return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "log kernel failed");
+}
+
+}  // namespace galp
+
+using namespace galp;
+
+extern "C" {
+
+GALP_API int galp_graph_set_mlp(galp_graph *g, int32_t hidden, const float *params, float cand_thresh) {
+    if (!g || !params) return GALP_EINVAL;
+    if (hidden != 8 && hidden != 16 && hidden != 32) return fail(g->ctx, GALP_EINVAL, "hidden must be 8, 16 or 32");
+    if (!(cand_thresh >= 0.f)) return fail(g->ctx, GALP_EINVAL, "cand_thresh must be >= 0");
+    int rc = learn_setup(g);
+    if (rc != GALP_OK) return rc;
+    LearnState *L = g->learn;
+    if (!L->d_params) {
+        if (hipMalloc((void **)&L->d_params, 4 * (size_t)mlp_num_params(32)) != hipSuccess)
+            return fail(g->ctx, GALP_ENOMEM, "parameter buffer");
+    }
+    if (hipMemcpy(L->d_params, params, 4 * (size_t)mlp_num_params(hidden), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "parameter upload failed");
+    L->H = hidden;
+    L->cand_thresh = cand_thresh;
+    return GALP_OK;
+}
+
+GALP_API int galp_graph_log_enable(galp_graph *g, int64_t capacity) {
+    if (!g || capacity < 0) return GALP_EINVAL;
+    int rc = learn_setup(g);
+    if (rc != GALP_OK) return rc;
+    LearnState *L = g->learn;
+    if (L->d_log_x) (void)hipFree(L->d_log_x);
+    if (L->d_log_y) (void)hipFree(L->d_log_y);
+    L->d_log_x = L->d_log_y = nullptr;
+    L->log_cap = 0;
+    if (capacity) {
+        if (hipMalloc((void **)&L->d_log_x, 16 * (size_t)capacity) != hipSuccess ||
+            hipMalloc((void **)&L->d_log_y, 4 * (size_t)capacity) != hipSuccess)
+            return fail(g->ctx, GALP_ENOMEM, "log buffer");
+        L->log_cap = capacity;
+    }
+    int64_t zero = 0;
+    if (hipMemcpy(L->d_log_n, &zero, 8, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "log reset failed");
+    return GALP_OK;
+}
+
+GALP_API int galp_graph_log_read(galp_graph *g, float *x, float *y, int64_t max, int64_t *count) {
+    if (!g || !count || !g->learn || !g->learn->d_log_n) return GALP_EINVAL;
+    LearnState *L = g->learn;
+    int64_t nlog = 0;
+    if (hipMemcpy(&nlog, L->d_log_n, 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "log readback failed");
+    nlog = std::min(nlog, L->log_cap);
+    *count = nlog;
+    const int64_t k = std::min(nlog, max);
+    if (k > 0 && x && y) {
+        if (hipMemcpy(x, L->d_log_x, 16 * (size_t)k, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(y, L->d_log_y, 4 * (size_t)k, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(g->ctx, GALP_EHIP, "log readback failed");
+    }
+    return GALP_OK;
+}
+
+GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *mx) {
+    if (!g || !deg || !mean || !mx) return GALP_EINVAL;
+    int rc = learn_setup(g);
+    if (rc != GALP_OK) return rc;
+    LearnState *L = g->learn;
+    float *tmp = nullptr;
+    if (hipMalloc((void **)&tmp, 4 * (size_t)std::max<int64_t>(g->n, 1)) != hipSuccess)
+        return fail(g->ctx, GALP_ENOMEM, "feature staging");
+    float *srcs[3] = {L->d_fdeg, L->d_fmean, L->d_fmax};
+    float *dsts[3] = {deg, mean, mx};
+    for (int k = 0; k < 3 && g->n; ++k) {
+        k_gather_feat<<<grid_cap(g, (g->n + 255) / 256, 32), 256, 0, g->stream>>>(srcs[k], g->d_perm, g->n, tmp);
+        if (hipMemcpyAsync(dsts[k], tmp, 4 * (size_t)g->n, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess) {
+            (void)hipFree(tmp);
+            return fail(g->ctx, GALP_EHIP, "feature readback failed");
+        }
+    }
+    (void)hipFree(tmp);
+    return GALP_OK;
+}
+
+}  // extern "C"

@@ -25,10 +25,15 @@
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "pipeline.hpp"
 
 namespace galp {
 
-const int64_t kBucketHi[NUM_BUCKETS] = {4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, INT64_MAX};
+const int64_t kBucketHi[NUM_BUCKETS] = {bucket_hi(0), bucket_hi(1), bucket_hi(2),  bucket_hi(3),
+                                        bucket_hi(4), bucket_hi(5), bucket_hi(6),  bucket_hi(7),
+                                        bucket_hi(8), bucket_hi(9), bucket_hi(10), bucket_hi(11)};
+static_assert(bucket_hi(0) == 4 && bucket_hi(4) == 64 && bucket_hi(5) == 128 && bucket_hi(10) == 4096,
+              "degree classes");
 const char *const kBucketName[NUM_BUCKETS] = {"g4",   "g8",   "g16",  "g32",  "g64",  "w128",
                                               "w256", "w512", "b1k",  "b2k",  "b4k",  "hub"};
 
@@ -164,66 +169,6 @@ k_hash_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
     block_count_add<4>(changed, counter, scratch);
 }
 
-// ---------------------------------------------------------------------------
-// Software pipeline shared by the table kernels.  A "work item" is a vertex
-// (LDS kernels) or a hub tile; Policy supplies
-//   struct Item { bool valid; int len; ... };
-//   Item item(int64_t i)               -- uniform per block (scalar loads)
-//   int64_t edge0(const Item &)        -- absolute index of the item's edge 0
-//   void process(const Item &, const int32_t (&lab)[EPT])
-// The driver keeps: labels of item t (being processed), neighbour ids of
-// item t+1, descriptor of item t+2; a period-2 unroll ping-pongs the label
-// registers so no in-flight load is ever copied.
-template <int T, int EPT, class Policy>
-__device__ __forceinline__ void pipelined(Policy &pol, const int32_t *__restrict__ nbr,
-                                          const int32_t *__restrict__ prev, int64_t n_items) {
-    using Item = typename Policy::Item;
-    const int64_t stride = gridDim.x;
-    int64_t i = blockIdx.x;
-    // Every lane issues every load (from a clamped, valid address) and
-    // selects afterwards: a load under a lane condition becomes a branch,
-    // and the compiler then drains all outstanding loads at the join.
-    // Loaded values are kept raw: masking them right after the load would
-    // wait for it.  Slots past an item's length hold junk that process()
-    // never reads (it checks pos < len).
-    auto load_nbr = [&](int32_t (&u)[EPT], const Item &it) {
-        const int64_t e0 = it.valid ? pol.edge0(it) : 0;
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int pos = k * T + (int)threadIdx.x;
-            const bool ok = it.valid && pos < it.len;
-            u[k] = __builtin_nontemporal_load(&nbr[ok ? e0 + pos : 0]);
-        }
-    };
-    auto gather = [&](int32_t (&lab)[EPT], const int32_t (&u)[EPT], const Item &it) {
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int pos = k * T + (int)threadIdx.x;
-            const bool ok = it.valid && pos < it.len;
-            lab[k] = prev[ok ? u[k] : 0];
-        }
-    };
-    Item s0 = pol.item(i), s1 = pol.item(i + stride), s2 = pol.item(i + 2 * stride);
-    int32_t u[EPT], labA[EPT], labB[EPT];
-    load_nbr(u, s0);
-    gather(labA, u, s0);
-    load_nbr(u, s1);
-    while (s0.valid) {
-        gather(labB, u, s1);    // labels of item t+1
-        load_nbr(u, s2);        // neighbours of item t+2
-        Item s3 = pol.item(i + 3 * stride);
-        pol.process(s0, labA);  // item t
-        s0 = s1; s1 = s2; s2 = s3; i += stride;
-        if (!s0.valid) break;
-        gather(labA, u, s1);
-        load_nbr(u, s2);
-        s3 = pol.item(i + 3 * stride);
-        pol.process(s0, labB);
-        s0 = s1; s1 = s2; s2 = s3; i += stride;
-    }
-    (void)n_items;
-}
-
 // Open-addressing insert into an LDS table of 2^BITS slots; returns the slot
 // and sets *created when this lane claimed an empty slot.  The caller
 // guarantees fewer keys than slots; the probe bound only guards the GPU.
@@ -324,6 +269,7 @@ struct LdsVertexPolicy {
     uint32_t *skey, *scnt, *sred;
     unsigned long long *sred64;
     int32_t *err;
+    int32_t *distinct;   // if set: distinct labels per vertex (learned dispatch feature)
     uint32_t changed;
 
     __device__ Item item(int64_t i) const {
@@ -346,15 +292,14 @@ struct LdsVertexPolicy {
     __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
         if (it.len == 0) return;
         uint32_t slot[EPT];
-        bool own[EPT];
+        bool own[EPT], creator[EPT];
         uint32_t maxc = 0;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * T + (int)threadIdx.x;
-            bool creator;
             slot[k] = lds_insert_chunk<CAPBITS, 3, false>(skey, scnt, nullptr, pos < it.len ? lab[k] : -1,
-                                                          (uint32_t)pos, &own[k], &creator, &maxc, err);
+                                                          (uint32_t)pos, &own[k], &creator[k], &maxc, err);
         }
         __syncthreads();
         maxc = block_max<T / kWave>(maxc, sred);
@@ -366,6 +311,12 @@ struct LdsVertexPolicy {
                 best = ((unsigned long long)(pos + 1) << 32) | (uint32_t)lab[k];
         }
         best = block_max<T / kWave>(best, sred64);
+        uint32_t ndist = 0;
+        if (distinct) {
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) ndist += creator[k];
+            ndist = block_sum<T / kWave>(ndist, sred);
+        }
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             if (own[k]) {
@@ -377,6 +328,7 @@ struct LdsVertexPolicy {
             const int32_t w = (int32_t)(uint32_t)best;
             next[it.v] = w;
             changed += (w != it.old);
+            if (distinct) distinct[it.v] = (int32_t)ndist;
         }
     }
 };
@@ -385,9 +337,9 @@ template <int T, int EPT, int CAPBITS, typename E>
 __global__ void __launch_bounds__(T)
 k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
            const int32_t *__restrict__ prev, int32_t *__restrict__ next,
-           const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+           const int32_t *__restrict__ vlist, int64_t v0, int64_t nv, const int64_t *__restrict__ nv_dev,
            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
-           int32_t *__restrict__ err) {
+           int32_t *__restrict__ err, int32_t *__restrict__ distinct) {
     static_assert((1 << CAPBITS) >= 2 * T * EPT, "load factor must stay <= 1/2");
     constexpr int CAP = 1 << CAPBITS;
     __shared__ uint32_t skey[CAP];
@@ -399,8 +351,10 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
         skey[s] = kEmpty;
         scnt[s] = 0;
     }
-    LdsVertexPolicy<T, EPT, CAPBITS, E> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err, 0u};
-    pipelined<T, EPT>(pol, nbr, prev, nv);
+    if (nv_dev) nv = *nv_dev;   // list built on the device this iteration
+    LdsVertexPolicy<T, EPT, CAPBITS, E> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err,
+                                            distinct, 0u};
+    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
     __syncthreads();
     block_count_add<T / kWave>(pol.changed, counter, sred);
 }
@@ -443,6 +397,7 @@ struct HubTilePolicy {
     uint32_t *out_key, *out_cnt, *out_last;
     int32_t *tile_idx;
     int32_t *err;
+    const int32_t *hub_sel;   // null: every hub; else only hubs with a non-zero flag
 
     __device__ Item item(int64_t t) const {
         Item it{false, 0, 0, 0, 1, 0, t};
@@ -453,6 +408,7 @@ struct HubTilePolicy {
             it.row0 = (uint32_t)((t - (int64_t)hub_tile0[h]) * (int64_t)kHubTile);   // tiles of a hub are consecutive
             it.e0 = rb + it.row0;
             it.len = (int)min((int64_t)kHubTile, re - it.e0);
+            if (hub_sel && !hub_sel[h]) it.len = 0;   // writes empty partition offsets
             it.P = (uint32_t)hub_nparts[h];
             it.idx = tile_idx_off[t];
             it.valid = true;
@@ -520,7 +476,8 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
            const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
            const int64_t *__restrict__ tile_idx_off, int64_t ntiles,
            uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last,
-           int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+           int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
+           const int32_t *__restrict__ hub_sel) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
     __shared__ uint32_t skey[CAP];
@@ -533,9 +490,10 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
         scnt[s] = 0;
         slast[s] = 0;
     }
-    HubTilePolicy<T, EPT, E> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles, skey,
-                                 scnt, slast,    hist,      out_key,  out_cnt,    out_last,     tile_idx, err};
-    pipelined<T, EPT>(pol, nbr, prev, ntiles);
+    HubTilePolicy<T, EPT, E> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles,  skey,
+                                 scnt, slast,    hist,      out_key,  out_cnt,    out_last,     tile_idx, err,
+                                 hub_sel};
+    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
 template <int T>
@@ -545,7 +503,8 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
             const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
             const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_cnt,
             const uint32_t *__restrict__ out_last, unsigned long long *__restrict__ hub_best,
-            const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+            const int32_t *__restrict__ stop, int32_t *__restrict__ err,
+            const int32_t *__restrict__ hub_sel, uint32_t *__restrict__ hub_distinct) {
     constexpr int CAP = 1 << kHubMergeCapBits;
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
@@ -563,6 +522,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     if (threadIdx.x == 0) nlist = 0;
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int h = item_hub[it];
+        if (hub_sel && !hub_sel[h]) continue;   // uniform per block
         const int p = item_part[it];
         const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
         // tiles in chunks of T: the chunk's slice bounds are loaded one tile
@@ -619,6 +579,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         if (threadIdx.x == 0) nlist = 0;
         best = block_max<T / kWave>(best, sred64);
         if (threadIdx.x == 0 && best) atomicMax(&hub_best[h], best);
+        if (threadIdx.x == 0 && hub_distinct) atomicAdd(&hub_distinct[h], nl);
     }
 }
 
@@ -627,13 +588,15 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
 template <typename E>
 __global__ void __launch_bounds__(256)
 k_hub_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ hub_vid,
-            int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ prev,
+            int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ hub_sel,
+            uint32_t *__restrict__ hub_distinct, int32_t *__restrict__ distinct, const int32_t *__restrict__ prev,
             int32_t *__restrict__ next, unsigned long long *__restrict__ counter,
             const int32_t *__restrict__ stop) {
     __shared__ uint32_t sred[256 / kWave];
     if (*stop) return;
     uint32_t changed = 0;
     for (int64_t h = (int64_t)blockIdx.x * 256 + threadIdx.x; h < nhubs; h += (int64_t)gridDim.x * 256) {
+        if (hub_sel && !hub_sel[h]) continue;
         const int64_t v = hub_vid[h];
         const unsigned long long pk = hub_best[h];
         const uint32_t last = (uint32_t)pk;
@@ -641,6 +604,10 @@ k_hub_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const in
         next[v] = w;
         changed += (w != prev[v]);
         hub_best[h] = 0ull;
+        if (hub_distinct) {
+            if (distinct) distinct[v] = (int32_t)hub_distinct[h];
+            hub_distinct[h] = 0u;
+        }
     }
     block_count_add<4>(changed, counter, sred);
 }
@@ -694,9 +661,15 @@ static void launch_lds(galp_graph *g, int bid, const int32_t *prev, int32_t *nex
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
-    k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-        (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop,
-        g->d_err);
+    int32_t *dist = g->learn ? g->learn->distinct_out(g) : nullptr;
+    if (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED)   // this iteration's list for the class
+        k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, next, g->learn->plist[bid], 0, r.nv, g->learn->d_plist_n + bid,
+            g->d_counter, g->d_stop, g->d_err, dist);
+    else
+        k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop,
+            g->d_err, dist);
     prof_end(g, &pl);
 }
 
@@ -707,21 +680,25 @@ static int hash_iteration(galp_graph *g) {
     // largest work first
     if (g->nhubs) {
         const BucketRange &r = g->bucket[B_HUB];
+        const int32_t *sel = (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) ? g->learn->d_hub_sel : nullptr;
+        uint32_t *hdist = g->learn ? g->learn->d_hub_distinct : nullptr;
         PendingLaunch pl;
         prof_begin(g, "hub_tile", 8.0 * (double)r.edges, &pl);
         k_hub_tile<256, E><<<grid_for(g, g->nhub_tiles, 2), 256, 0, g->stream>>>(
             (const E *)g->d_off, g->d_nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-            g->d_tile_idx_off, g->nhub_tiles, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop, g->d_err);
+            g->d_tile_idx_off, g->nhub_tiles, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop, g->d_err,
+            sel);
         prof_end(g, &pl);
         prof_begin(g, "hub_merge", 0.0, &pl);
         k_hub_merge<256><<<grid_for(g, g->nhub_items, 3), 256, 0, g->stream>>>(
             g->d_item_hub, g->d_item_part, g->nhub_items, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
-            g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err);
+            g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, sel,
+            hdist);
         prof_end(g, &pl);
         prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)r.nv + 16.0 * (double)r.nv, &pl);
         k_hub_final<E><<<grid_for(g, (g->nhubs + 255) / 256, 4), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, prev, next, g->d_counter,
-            g->d_stop);
+            (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, sel, hdist,
+            g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter, g->d_stop);
         prof_end(g, &pl);
     }
     launch_lds<512, 8, 13, E>(g, B_B4K, prev, next, 2);

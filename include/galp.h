@@ -181,6 +181,49 @@ GALP_API int galp_profile_read(galp_graph *g, galp_kernel_stat *out,
                                int32_t max, int32_t *count);
 GALP_API int galp_profile_reset(galp_graph *g);
 
+/* ---- learned dispatch: the distinct-label predictor ----------------------
+ * A 4 -> H -> H -> 1 MLP (ReLU, ReLU, linear; fp32; H in {8, 16, 32}) with
+ * parameters laid out as W1[H][4], b1[H], W2[H][H], b2[H], W3[H], b3.
+ * Inputs per vertex: log2(1+degree), log2(1+mean neighbour degree),
+ * log2(1+max neighbour degree), log2(1+distinct labels of the previous
+ * iteration) or -1 before the first; target log2(1+distinct labels).
+ * No reference counterpart (the reference dispatches by -l only,
+ * lfht.cuh:127-165).  All computation runs in HIP kernels. */
+GALP_API int64_t galp_mlp_num_params(int32_t hidden);
+/* He-initialised parameters from a seed (host only). */
+GALP_API int galp_mlp_init(int32_t hidden, uint64_t seed, float *params);
+/* y_hat[n] for x[n][4] (host arrays). */
+GALP_API int galp_mlp_predict(galp_ctx *ctx, int32_t hidden, const float *params,
+                              const float *x, int64_t n, float *yhat);
+/* Loss (mean of 0.5 (y_hat - y)^2) and its gradient over x[n][4], y[n]. */
+GALP_API int galp_mlp_loss_grad(galp_ctx *ctx, int32_t hidden, const float *params,
+                                const float *x, const float *y, int64_t n,
+                                float *grad, float *loss);
+/* Adam over `epochs` passes of minibatches of `batch` samples (seeded
+ * shuffle); params are updated in place; final_loss = loss over all n. */
+GALP_API int galp_mlp_train(galp_ctx *ctx, int32_t hidden, float *params,
+                            const float *x, const float *y, int64_t n,
+                            int32_t epochs, int32_t batch, float lr,
+                            uint64_t seed, float *final_loss);
+
+/* Attaches a trained network to a graph uploaded with dispatch = learned:
+ * vertices of degree > 64 whose predicted distinct-label count is <=
+ * cand_thresh take the register-candidate path, the others the degree-class
+ * kernels; a misprediction falls back to the latter within the iteration. */
+GALP_API int galp_graph_set_mlp(galp_graph *g, int32_t hidden, const float *params, float cand_thresh);
+/* Logs one training sample per vertex of degree > 64 and iteration (inputs
+ * from the previous iteration's distinct count, target this iteration's),
+ * up to `capacity` samples; galp_graph_log_read copies them out. */
+GALP_API int galp_graph_log_enable(galp_graph *g, int64_t capacity);
+GALP_API int galp_graph_log_read(galp_graph *g, float *x, float *y, int64_t max, int64_t *count);
+/* Raw static features per vertex (original numbering): degree, mean and max
+ * neighbour degree, computed for vertices of degree > 64 (0 otherwise). */
+GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *mx);
+/* Counters of the last iteration's learned dispatch: [0..12) class list
+ * sizes, [12] candidate tiles, [13] candidate overflows, [14] vertices
+ * routed to candidates. */
+GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max, int32_t *count);
+
 /* Message of the last failure on ctx; "" if none. */
 GALP_API const char *galp_last_error(galp_ctx *ctx);
 

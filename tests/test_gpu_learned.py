@@ -1,0 +1,227 @@
+"""The learned dispatcher on the GPU, through the C-ABI.
+
+- the MLP's HIP inference, loss and gradient against the oracle's float64
+  forward/backward (itself pinned by finite differences in test_oracle.py);
+- HIP training lowers the loss on a target the network can fit;
+- the static features bit-exact against the oracle;
+- the logged distinct-label counts exact against the oracle;
+- labels bit-exact against the oracle after every iteration whatever the
+  network predicts: everything to the candidate path (thresholds far above
+  any count, so multi-label rows overflow and take the spill path), nothing,
+  and seeded random networks in between.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from galp_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+LEARNED = _abi.DISPATCH_LEARNED
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_x(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.empty((n, 4), np.float32)
+    x[:, 0] = rng.uniform(6, 20, n)
+    x[:, 1] = rng.uniform(0, 12, n)
+    x[:, 2] = x[:, 1] + rng.uniform(0, 8, n)
+    x[:, 3] = np.where(rng.random(n) < 0.2, -1.0, rng.uniform(0, 14, n))
+    return x
+
+
+@pytest.mark.parametrize("hidden", [8, 16, 32])
+def test_mlp_predict_and_grad_match_oracle(ctx, hidden):
+    say(f"[mlp] hidden={hidden}")
+    p = _abi.mlp_init(hidden, 7 + hidden)
+    x = rand_x(5000, hidden)
+    y = (0.5 * x[:, 0] + 0.1 * x[:, 3]).astype(np.float32)
+    yhat = _abi.mlp_predict(ctx, hidden, p, x)
+    loss_ref, yhat_ref, grad_ref = oracle.mlp_loss_grad(hidden, p, x, y)
+    # fp32 forward of <= 32 + 32*32 products of O(10) terms: 1e-4 relative
+    np.testing.assert_allclose(yhat, yhat_ref, rtol=1e-4, atol=1e-4)
+    loss, grad = _abi.mlp_loss_grad(ctx, hidden, p, x, y)
+    assert abs(loss - loss_ref) <= 1e-4 * max(1.0, abs(loss_ref))
+    # fp32 sums of 5000 per-sample terms: 1e-3 of the gradient's scale
+    scale = np.abs(grad_ref).max()
+    np.testing.assert_allclose(grad, grad_ref, rtol=1e-3, atol=1e-3 * scale)
+
+
+def test_mlp_train_lowers_loss(ctx):
+    hidden = 16
+    x = rand_x(20000, 3)
+    # a smooth target of the kind the dispatcher learns: distinct labels grow
+    # with the degree and track the previous count when there is one
+    y = np.where(x[:, 3] >= 0, 0.8 * x[:, 3] + 0.1 * x[:, 0], 0.9 * x[:, 0] - 0.2 * x[:, 1]).astype(np.float32)
+    p0 = _abi.mlp_init(hidden, 11)
+    loss0, _, _ = oracle.mlp_loss_grad(hidden, p0, x, y)
+    p1, loss1 = _abi.mlp_train(ctx, hidden, p0, x, y, epochs=30, batch=256, lr=3e-3, seed=5)
+    ref1, _, _ = oracle.mlp_loss_grad(hidden, p1, x, y)
+    say(f"[mlp] train loss {loss0:.4f} -> {loss1:.4f} (oracle {ref1:.4f})")
+    assert abs(loss1 - ref1) <= 1e-3 * max(1.0, ref1)
+    assert ref1 < 0.05 * loss0
+    p2, loss2 = _abi.mlp_train(ctx, hidden, p0, x, y, epochs=30, batch=256, lr=3e-3, seed=5)
+    assert np.array_equal(p1, p2), "training must be deterministic for a seed"
+
+
+def big_graph(seed=0):
+    """Hubs (multi-tile vertices of every size up to 40k), every LDS class,
+    and small vertices, with repeated neighbours."""
+    rng = np.random.default_rng(seed)
+    n = 60000
+    rows = [[] for _ in range(n)]
+    for h, d in ((0, 40000), (1, 9000), (2, 5000), (3, 3000), (4, 1500), (5, 1100), (6, 700), (7, 300)):
+        for u in rng.choice(np.arange(8, n), size=d, replace=False):
+            rows[h].append(int(u))
+            rows[int(u)].append(h)
+    for v in range(8, n):
+        k = int(rng.integers(0, 12))
+        for u in rng.integers(8, n, size=k):
+            rows[v].append(int(u))
+            rows[int(u)].append(v)
+    for v in range(8, 2000):   # a few more W128..B1K vertices
+        for u in rng.integers(8, n, size=int(rng.integers(60, 600))):
+            rows[v].append(int(u))
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(r) for r in rows])
+    nbr = np.array([u for r in rows for u in r], np.int32)
+    return off, nbr
+
+
+@pytest.mark.parametrize("reorder", [1, 0])
+def test_features_bit_exact(ctx, reorder):
+    off, nbr = big_graph(1)
+    d, mean, mx = oracle.vertex_features(off, nbr)
+    big = np.diff(off) > 64
+    with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, reorder=reorder) as g:
+        gd, gmean, gmx = g.features()
+    assert np.array_equal(gd[big], d[big])
+    assert np.array_equal(gmean[big], mean[big])
+    assert np.array_equal(gmx[big], mx[big])
+    assert not gd[~big].any() and not gmean[~big].any() and not gmx[~big].any()
+
+
+@pytest.mark.parametrize("reorder", [1, 0])
+def test_logged_distinct_counts_exact(ctx, reorder):
+    off, nbr = big_graph(2)
+    n = len(off) - 1
+    big = np.nonzero(np.diff(off) > 64)[0]
+    prev = np.arange(n, dtype=np.int32)
+    with _abi.Graph(ctx, off, nbr, reorder=reorder) as g:   # plain dispatch logs too
+        g.log_enable(10 * len(big))
+        g.init()
+        want_prev = None
+        for it in range(3):
+            dist = oracle.distinct_counts(off, nbr, prev)[big]
+            g.step()
+            x, y = g.log_read()
+            assert len(x) == (it + 1) * len(big)
+            xs, ys = x[it * len(big):], y[it * len(big):]
+            got = np.rint(np.exp2(ys.astype(np.float64)) - 1).astype(np.int64)
+            assert np.array_equal(np.sort(got), np.sort(dist)), it
+            if want_prev is None:
+                assert np.all(xs[:, 3] == -1.0)
+            else:
+                gotp = np.rint(np.exp2(xs[:, 3].astype(np.float64)) - 1).astype(np.int64)
+                assert np.array_equal(np.sort(gotp), np.sort(want_prev)), it
+            want_prev = dist
+            prev, _ = oracle.lp_iterate(off, nbr, prev)
+
+
+def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, reorder=1):
+    say(f"[learned] {tag} n={len(off) - 1} m={len(nbr)} thresh={thresh} reorder={reorder}")
+    n = len(off) - 1
+    prev = np.arange(n, dtype=np.int32) if prev is None else prev
+    routed = overflowed = 0
+    with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, reorder=reorder) as g:
+        g.set_mlp(hidden, params, thresh)
+        g.set_labels(prev)
+        for it in range(niter):
+            ref, cnt = oracle.lp_iterate(off, nbr, prev)
+            st = g.step()
+            got = g.labels()
+            if not np.array_equal(got, ref):
+                bad = np.nonzero(got != ref)[0]
+                raise AssertionError(f"{tag} iteration {it + 1}: {len(bad)} labels differ, first at vertex "
+                                     f"{bad[0]} (deg {off[bad[0] + 1] - off[bad[0]]}): gpu {got[bad[0]]} "
+                                     f"oracle {ref[bad[0]]}")
+            assert st["updated"] == cnt, (tag, it + 1, st["updated"], cnt)
+            ds = g.dispatch_stats()
+            routed += int(ds[14])
+            overflowed += int(ds[13])
+            assert st["overflowed"] == ds[13]
+            prev = ref
+            if st["stopped"]:
+                break
+    return routed, overflowed
+
+
+def const_net(hidden, value):
+    """A network whose output is the constant `value` (all weights 0)."""
+    p = np.zeros(_abi.mlp_num_params(hidden), np.float32)
+    p[-1] = value
+    return p
+
+
+@pytest.mark.parametrize("reorder", [1, 0])
+def test_learned_everything_to_candidates(ctx, reorder):
+    off, nbr = big_graph(3)
+    # predicted 0 distinct labels: every big vertex tries the candidate path;
+    # rows with more than 8 labels overflow to the class kernels
+    routed, overflowed = run_learned(ctx, off, nbr, 6, "all-cand", const_net(8, 0.0), 8, 4.0, reorder=reorder)
+    assert routed > 0 and overflowed > 0
+
+
+@pytest.mark.parametrize("reorder", [1, 0])
+def test_learned_nothing_to_candidates(ctx, reorder):
+    off, nbr = big_graph(4)
+    routed, overflowed = run_learned(ctx, off, nbr, 4, "no-cand", const_net(8, 30.0), 8, 4.0, reorder=reorder)
+    assert routed == 0 and overflowed == 0
+
+
+def test_learned_clustered_labels_hit_candidates(ctx):
+    # few distinct labels everywhere: the candidate path resolves hubs and
+    # big vertices without overflow
+    off, nbr = big_graph(5)
+    n = len(off) - 1
+    rng = np.random.default_rng(9)
+    prev = rng.integers(0, 6, size=n).astype(np.int32)
+    routed, overflowed = run_learned(ctx, off, nbr, 3, "clustered", const_net(16, 0.0), 16, 8.0, prev=prev)
+    assert routed > 0 and overflowed == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_learned_random_networks(ctx, seed):
+    off, nbr = _abi.gen_rmat(16, 16, seed=seed)
+    p = _abi.mlp_init(32, seed)
+    run_learned(ctx, off, nbr, 8, f"rmat16-rand{seed}", p, 32, 3.0)
+
+
+def test_learned_device_loop(ctx):
+    off, nbr = _abi.gen_rmat(15, 16, seed=4)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    with _abi.Graph(ctx, off, nbr, dispatch=LEARNED) as g:
+        g.set_mlp(8, const_net(8, 1.0), 4.0)
+        labels, res = g.run(10)
+    assert res["iterations"] == len(per)
+    assert np.array_equal(labels, final)
+
+
+def test_learned_needs_a_network(ctx):
+    off = np.array([0, 1, 2], np.int64)
+    with _abi.Graph(ctx, off, np.array([1, 0], np.int32), dispatch=LEARNED) as g:
+        g.init()
+        with pytest.raises(_abi.GalpError) as e:
+            g.step()
+        assert e.value.code == -6
+        with pytest.raises(_abi.GalpError) as e:
+            g.set_mlp(12, np.zeros(_abi.mlp_num_params(12), np.float32), 1.0)
+        assert e.value.code == -1
