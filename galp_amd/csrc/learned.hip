@@ -88,6 +88,14 @@ __global__ void k_reset_ctrs(int64_t *ctr) {
     if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
 }
 
+// One block routes kDispChunk consecutive big vertices.  Appends are
+// aggregated per block: the block counts its entries per list in LDS, takes
+// one global atomicAdd per list, then hands out positions from LDS cursors.
+// (Same-address global atomics serialise at the L2; one per wave and list
+// cost ~100 us per iteration on RMAT-22, one per block and list does not.)
+constexpr int kDispChunk = 1024;
+enum : uint8_t { kCodeCand = NUM_BUCKETS, kCodeCandMt, kCodeHub, kCodeNone };
+
 template <int H, typename E>
 __global__ void __launch_bounds__(256)
 k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t big_v0, int64_t big_n,
@@ -98,56 +106,79 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
            int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len, int32_t *__restrict__ cw_pslot,
            int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr, const int32_t *__restrict__ stop) {
     __shared__ float sp[mlp_num_params(H)];
+    __shared__ uint8_t scode[kDispChunk];
+    __shared__ int32_t sv[kDispChunk], smt[kDispChunk];
+    __shared__ int64_t sb[kDispChunk], sd[kDispChunk];
+    __shared__ unsigned long long scount[NUM_BUCKETS + 2];   // class lists, candidate tiles, candidate vertices
     if (*stop) return;
     for (int i = threadIdx.x; i < mlp_num_params(H); i += 256) sp[i] = params[i];
+    if (threadIdx.x < NUM_BUCKETS + 2) scount[threadIdx.x] = 0;
     __syncthreads();
-    for (int64_t base = (int64_t)blockIdx.x * 256; base < big_n; base += (int64_t)gridDim.x * 256) {
-        const int64_t i = base + threadIdx.x;
-        const bool ok = i < big_n;
-        int64_t v = 0, b = 0, d = 0;
-        bool cand = false;
-        int cls = -1, mt = -1;
-        if (ok) {
-            v = vlist ? (int64_t)vlist[big_v0 + i] : big_v0 + i;
-            b = (int64_t)off[v];
-            d = (int64_t)off[v + 1] - b;
-            float x[4];
-            mlp_inputs(fdeg[v], fmean[v], fmax[v], dprev[v], x);
-            const float y = mlp_forward<H>(sp, x[0], x[1], x[2], x[3]);
-            cand = exp2f(y) - 1.f <= cand_thresh;
-            cls = bucket_of_deg(d);
-            mt = d > kCandTile ? mt_of[v] : -1;
-        }
-        // candidates of one tile
-        const int64_t pos = wave_append(ctr + kCtrCandTiles, ok && cand && mt < 0);
-        if (pos >= 0) {
-            cw_v[pos] = (int32_t)v;
-            cw_e0[pos] = b;
-            cw_len[pos] = (int32_t)d;
-            cw_pslot[pos] = -1;
-        }
-        (void)wave_append(ctr + kCtrCandVerts, ok && cand);
-        // multi-tile vertices: ceil(deg / kCandTile) tiles with partial slots
+    const int64_t i0 = (int64_t)blockIdx.x * kDispChunk;
+    const int nloc = (int)min((int64_t)kDispChunk, big_n - i0);
+    // phase 1: predict, classify, count
+    for (int k = threadIdx.x; k < nloc; k += 256) {
+        const int64_t i = i0 + k;
+        const int64_t v = vlist ? (int64_t)vlist[big_v0 + i] : big_v0 + i;
+        const int64_t b = (int64_t)off[v];
+        const int64_t d = (int64_t)off[v + 1] - b;
+        float x[4];
+        mlp_inputs(fdeg[v], fmean[v], fmax[v], dprev[v], x);
+        const float y = mlp_forward<H>(sp, x[0], x[1], x[2], x[3]);
+        const bool cand = exp2f(y) - 1.f <= cand_thresh;
+        const int cls = bucket_of_deg(d);
+        const int mt = d > kCandTile ? mt_of[v] : -1;
+        uint8_t code;
         if (mt >= 0) {
             mt_cand[mt] = cand ? 1 : 0;
             if (mt < nhubs) hub_sel[mt] = cand ? 0 : 1;
-            if (cand) {
-                const int nt = (int)((d + kCandTile - 1) / kCandTile);
-                const int64_t p0 = (int64_t)atomicAdd((unsigned long long *)(ctr + kCtrCandTiles),
-                                                      (unsigned long long)nt);
-                for (int t = 0; t < nt; ++t) {
-                    cw_v[p0 + t] = (int32_t)v;
-                    cw_e0[p0 + t] = b + (int64_t)t * kCandTile;
-                    cw_len[p0 + t] = (int32_t)min((int64_t)kCandTile, d - (int64_t)t * kCandTile);
-                    cw_pslot[p0 + t] = mt_ctile0[mt] + t;
-                }
-            }
+            code = cand ? (uint8_t)kCodeCandMt : cls < B_HUB ? (uint8_t)cls : (uint8_t)kCodeHub;
+        } else {
+            code = cand ? (uint8_t)kCodeCand : (uint8_t)cls;
         }
-        // plain classes W128..B4K
-#pragma unroll
-        for (int c = B_W128; c < B_HUB; ++c) {
-            const int64_t q = wave_append(ctr + c, ok && !cand && cls == c);
-            if (q >= 0) plist[c][q] = (int32_t)v;
+        if (code < NUM_BUCKETS) atomicAdd(&scount[code], 1ull);
+        if (code == kCodeCand) atomicAdd(&scount[NUM_BUCKETS], 1ull);
+        if (code == kCodeCandMt) atomicAdd(&scount[NUM_BUCKETS], (unsigned long long)((d + kCandTile - 1) / kCandTile));
+        if (cand) atomicAdd(&scount[NUM_BUCKETS + 1], 1ull);
+        scode[k] = code;
+        sv[k] = (int32_t)v;
+        smt[k] = mt;
+        sb[k] = b;
+        sd[k] = d;
+    }
+    __syncthreads();
+    // phase 2: one global atomic per non-empty list; LDS cursors start at its base
+    if (threadIdx.x < NUM_BUCKETS + 2) {
+        const int c = threadIdx.x;
+        const int gc = c < NUM_BUCKETS ? c : c == NUM_BUCKETS ? kCtrCandTiles : kCtrCandVerts;
+        const unsigned long long cnt = scount[c];
+        scount[c] = cnt ? atomicAdd((unsigned long long *)(ctr + gc), cnt) : 0ull;
+    }
+    __syncthreads();
+    // phase 3: write the entries
+    for (int k = threadIdx.x; k < nloc; k += 256) {
+        const uint8_t code = scode[k];
+        const int32_t v = sv[k];
+        if (code < NUM_BUCKETS) {
+            const unsigned long long q = atomicAdd(&scount[code], 1ull);
+            plist[code][q] = v;
+        } else if (code == kCodeCand) {
+            const unsigned long long q = atomicAdd(&scount[NUM_BUCKETS], 1ull);
+            cw_v[q] = v;
+            cw_e0[q] = sb[k];
+            cw_len[q] = (int32_t)sd[k];
+            cw_pslot[q] = -1;
+        } else if (code == kCodeCandMt) {
+            const int64_t d = sd[k], b = sb[k];
+            const int nt = (int)((d + kCandTile - 1) / kCandTile);
+            const unsigned long long q = atomicAdd(&scount[NUM_BUCKETS], (unsigned long long)nt);
+            const int32_t ps = mt_ctile0[smt[k]];
+            for (int t = 0; t < nt; ++t) {
+                cw_v[q + t] = v;
+                cw_e0[q + t] = b + (int64_t)t * kCandTile;
+                cw_len[q + t] = (int32_t)min((int64_t)kCandTile, d - (int64_t)t * kCandTile);
+                cw_pslot[q + t] = ps + t;
+            }
         }
     }
 }
@@ -650,7 +681,8 @@ int learn_reset_iteration(galp_graph *g) {
 
 template <int H, typename E>
 static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl) {
-    k_dispatch<H, E><<<grid_cap(g, (L->big_n + 255) / 256, 8), 256, 0, g->stream>>>(
+    if (!L->big_n) return;
+    k_dispatch<H, E><<<(int)((L->big_n + kDispChunk - 1) / kDispChunk), 256, 0, g->stream>>>(
         (const E *)g->d_off, g->d_vlist, L->big_v0, L->big_n, L->d_params, L->d_fdeg, L->d_fmean, L->d_fmax,
         L->d_dist[L->dcur], L->cand_thresh, g->nhubs, L->d_mt_of, L->d_mt_ctile0, L->d_mt_cand, L->d_hub_sel, L->d_cw_v,
         L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, dpl, L->d_plist_n, g->d_stop);

@@ -347,11 +347,12 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
     __shared__ uint32_t sred[T / kWave];
     __shared__ unsigned long long sred64[T / kWave];
     if (*stop) return;
+    if (nv_dev) nv = *nv_dev;   // list built on the device this iteration
+    if ((int64_t)blockIdx.x >= nv) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
         scnt[s] = 0;
     }
-    if (nv_dev) nv = *nv_dev;   // list built on the device this iteration
     LdsVertexPolicy<T, EPT, CAPBITS, E> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err,
                                             distinct, 0u};
     pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
@@ -408,7 +409,7 @@ struct HubTilePolicy {
             it.row0 = (uint32_t)((t - (int64_t)hub_tile0[h]) * (int64_t)kHubTile);   // tiles of a hub are consecutive
             it.e0 = rb + it.row0;
             it.len = (int)min((int64_t)kHubTile, re - it.e0);
-            if (hub_sel && !hub_sel[h]) it.len = 0;   // writes empty partition offsets
+            if (hub_sel && !hub_sel[h]) it.len = 0;   // processed as a no-op
             it.P = (uint32_t)hub_nparts[h];
             it.idx = tile_idx_off[t];
             it.valid = true;
@@ -418,6 +419,7 @@ struct HubTilePolicy {
     __device__ int64_t edge0(const Item &it) const { return it.e0; }
 
     __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        if (it.len == 0) return;   // hub left to the candidate path: the merge skips it too
         uint32_t slot[EPT];
         bool own[EPT], creator[EPT];
         uint32_t maxc = 0;
