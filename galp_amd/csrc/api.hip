@@ -119,6 +119,7 @@ static int dev_alloc(galp_graph *g, T **p, int64_t count) {
 
 static void dev_free_all(galp_graph *g) {
     learn_free(g);
+    sort_free(g);
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
                     g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
@@ -309,8 +310,11 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     // ---- memory check (check_gpu_mem, main.cu:51-64) --------------------
     {
         const int64_t esz = g->off64 ? 8 : 4;
-        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 8 * g->nhub_items + 32 * g->nhubs +
-                       (12 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx + 4096;
+        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
+        if (g->opts.method == GALP_METHOD_HASH)
+            need += 8 * g->nhub_items + (8 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
+        else   // compact hub rows: keys and positions, twice; histograms and their scan
+            need += 16 * g->bucket[B_HUB].edges + 12 * 256 * g->nhub_tiles;
         if (g->reorder) need += 8 * n + 4 * m + 8 * (n + 1);  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;
@@ -374,14 +378,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_hub_best, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_tile0, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_ntiles, g->nhubs));
-        GALP_TRY(dev_alloc(g, &g->d_item_hub, g->nhub_items));
-        GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
         GALP_TRY(dev_alloc(g, &g->d_tile_hub, g->nhub_tiles));
-        GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
-        GALP_TRY(dev_alloc(g, &g->d_tile_idx, ntile_idx));
-        GALP_TRY(dev_alloc(g, &g->d_out_key, g->nhub_tiles * kHubTile));
-        GALP_TRY(dev_alloc(g, &g->d_out_cnt, g->nhub_tiles * kHubTile));
-        GALP_TRY(dev_alloc(g, &g->d_out_last, g->nhub_tiles * kHubTile));
         auto up = [&](void *dst, const void *src, size_t bytes) {
             return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
         };
@@ -389,11 +386,25 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_HIP(ctx, up(g->d_hub_nparts, hub_nparts.data(), 4 * hub_nparts.size()));
         GALP_HIP(ctx, up(g->d_hub_tile0, hub_tile0.data(), 4 * hub_tile0.size()));
         GALP_HIP(ctx, up(g->d_hub_ntiles, hub_ntiles.data(), 4 * hub_ntiles.size()));
-        GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
-        GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
         GALP_HIP(ctx, up(g->d_tile_hub, tile_hub.data(), 4 * tile_hub.size()));
-        GALP_HIP(ctx, up(g->d_tile_idx_off, tile_idx_off.data(), 8 * tile_idx_off.size()));
+        if (g->opts.method == GALP_METHOD_HASH) {   // tile outputs and merge items of the hash hub path
+            GALP_TRY(dev_alloc(g, &g->d_item_hub, g->nhub_items));
+            GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
+            GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
+            GALP_TRY(dev_alloc(g, &g->d_tile_idx, ntile_idx));
+            GALP_TRY(dev_alloc(g, &g->d_out_key, g->nhub_tiles * kHubTile));
+            GALP_TRY(dev_alloc(g, &g->d_out_cnt, g->nhub_tiles * kHubTile));
+            GALP_TRY(dev_alloc(g, &g->d_out_last, g->nhub_tiles * kHubTile));
+            GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
+            GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
+            GALP_HIP(ctx, up(g->d_tile_idx_off, tile_idx_off.data(), 8 * tile_idx_off.size()));
+        }
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
+    }
+    if (g->opts.method == GALP_METHOD_SORT) {
+        std::vector<int64_t> hub_deg;
+        for (int32_t x : hub_vertices) hub_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
+        GALP_TRY(sort_setup(g, hub_deg));
     }
     GALP_HIP(ctx, hipStreamSynchronize(g->stream));
     if (g->opts.dispatch == GALP_DISPATCH_LEARNED) GALP_TRY(learn_setup(g));
@@ -412,8 +423,11 @@ static int launch_iteration(galp_graph *g) {
         GALP_TRY(launch_hash_iteration(g));
         if (g->learn) GALP_TRY(learn_after_iteration(g));
         break;
+    case GALP_METHOD_SORT:
+        GALP_TRY(launch_sort_iteration(g));
+        break;
     default:
-        return fail(g->ctx, GALP_EUNSUPPORTED, "method " + std::to_string(g->opts.method) + " is not built yet");
+        return fail(g->ctx, GALP_EUNSUPPORTED, "method " + std::to_string(g->opts.method) + " is not built");
     }
     return launch_iter_end(g);
 }
@@ -498,7 +512,7 @@ GALP_API int galp_graph_upload(galp_ctx *ctx, const galp_csr_view *csr, const ga
     if (o.policy == 3) return fail(ctx, GALP_EUNSUPPORTED, "policy 3 (CPU-GPU hybrid) is not part of the one-GPU build");
     if (o.policy == 5 || (o.policy == 4 && o.method == GALP_METHOD_HASH))
         return fail(ctx, GALP_EUNSUPPORTED, "multi-GPU policies are not part of the one-GPU build");
-    if (o.policy != GALP_POLICY_INCORE)
+    if (o.policy != GALP_POLICY_INCORE && o.policy != GALP_POLICY_LI)
         return fail(ctx, GALP_EUNSUPPORTED, "policy " + std::to_string(o.policy) + " is not built yet");
     if (o.variant < 0 || o.variant > 3) return fail(ctx, GALP_EINVAL, "variant (-l) must be 0..3");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, GALP_EHIP, "hipSetDevice failed");

@@ -119,6 +119,19 @@ struct LearnState {
     int64_t *d_log_n = nullptr;
     int32_t *distinct_out(const galp_graph *) const { return d_dist[dcur ^ 1]; }
 };
+// State of the sort method's hub path: a compact buffer of the hub rows
+// (hub h at [hoff[h], hoff[h+1])), ping-pong keys and positions, and the
+// per-(hub, digit, tile) histograms and their scan.
+struct SortState {
+    int passes = 0;                    // 8-bit LSD passes for labels < n
+    int64_t eh = 0;                    // hub edges
+    int64_t *d_hoff = nullptr;
+    uint32_t *d_kA = nullptr, *d_kB = nullptr, *d_pA = nullptr, *d_pB = nullptr;
+    uint32_t *d_hist = nullptr;
+    int64_t *d_scan = nullptr, *d_bsum = nullptr;
+    int64_t nscan = 0, nbsum = 0;
+};
+
 constexpr int kCtrCandTiles = NUM_BUCKETS;
 constexpr int kCtrOverflow = NUM_BUCKETS + 1;
 constexpr int kCtrCandVerts = NUM_BUCKETS + 2;
@@ -180,6 +193,7 @@ struct galp_graph {
 
     galp::Profiler prof;
     galp::LearnState *learn = nullptr;
+    galp::SortState *sort = nullptr;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
     std::vector<int64_t> big_deg;
 };
@@ -194,6 +208,11 @@ int fail(galp_ctx *ctx, int code, const std::string &msg);
 int launch_hash_iteration(galp_graph *g);
 // Iteration bookkeeping: records the change count, sets the stop flag.
 int launch_iter_end(galp_graph *g);
+
+// Sort method (lp_sort.hip).
+int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg);
+void sort_free(galp_graph *g);
+int launch_sort_iteration(galp_graph *g);
 
 // Learned dispatch (learned.hip).
 int learn_setup(galp_graph *g);

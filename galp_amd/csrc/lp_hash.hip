@@ -26,6 +26,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "pipeline.hpp"
+#include "group_kernel.hpp"
 
 namespace galp {
 
@@ -93,81 +94,10 @@ __device__ __forceinline__ uint32_t group_best(int32_t L) {
     return best;
 }
 
-// One wave holds 64/G vertices per round; rounds are software-pipelined:
-// while round t is resolved, the labels of round t+1 and the neighbour ids
-// of round t+2 are loading, and the offsets of round t+3.
-template <int G, int PEEL, bool VLIST, typename E>
-__global__ void __launch_bounds__(256)
-k_hash_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
-             const int32_t *__restrict__ prev, int32_t *__restrict__ next,
-             const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
-             unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
-    __shared__ uint32_t scratch[256 / kWave];
-    if (*stop) return;
-    const int p = lane_id() & (G - 1);
-    constexpr int64_t per_block = 256 / G;
-    const int64_t stride = (int64_t)gridDim.x * per_block;
-    // Raw offsets stay in the pipeline registers until the next stage uses
-    // them (any arithmetic on a just-loaded value would wait for it), and
-    // every load is issued by every lane from a clamped address, so no load
-    // sits under a branch (see pipelined()).
-    struct St {
-        int64_t v;
-        E ob, oe;     // off[v], off[v+1], raw
-        bool ok;      // i < nv
-    };
-    auto state = [&](int64_t i) {
-        St s;
-        s.ok = i < nv;
-        const int64_t ic = s.ok ? i : 0;
-        if constexpr (VLIST) s.v = (int64_t)vlist[v0 + ic];
-        else s.v = v0 + ic;
-        s.ob = off[s.v];
-        s.oe = off[s.v + 1];
-        return s;
-    };
-    auto has = [&](const St &s) { return s.ok && (int64_t)p < (int64_t)(s.oe - s.ob); };
-    auto ldn = [&](const St &s) -> int32_t {
-        return __builtin_nontemporal_load(&nbr[has(s) ? (int64_t)s.ob + p : 0]);
-    };
-    auto gat = [&](int32_t u, const St &s) -> int32_t { return prev[has(s) ? u : 0]; };
-    uint32_t changed = 0;
-    auto resolve = [&](const St &s, int32_t Lraw, int32_t old) {
-        const int32_t L = has(s) ? Lraw : -1;
-        const uint32_t best = group_best<G, PEEL>(L);
-        const int32_t win = __shfl(L, (int)(best & 0xffu), G);
-        if (p == 0 && s.ok && s.oe != s.ob) {
-            next[s.v] = win;
-            changed += (win != old);
-        }
-    };
-    int64_t base = (int64_t)blockIdx.x * per_block;   // uniform round index
-    int64_t i = base + threadIdx.x / G;               // this lane's vertex index
-    // Four state slots rotate through the stages; the loop is unrolled by
-    // the rotation period so no register holding an in-flight load is ever
-    // copied (a copy would wait for it).
-    St sa = state(i), sb = state(i + stride), sc = state(i + 2 * stride), sd;
-    int32_t u = ldn(sa);
-    int32_t LA = gat(u, sa), oA = prev[sa.v], LB, oB;
-    u = ldn(sb);
-#define GALP_GROUP_STAGE(S0, S1, S2, S3, LC, OC, LN, ON) \
-    LN = gat(u, S1);                                     \
-    ON = prev[S1.v];                                     \
-    u = ldn(S2);                                         \
-    S3 = state(i + 3 * stride);                          \
-    resolve(S0, LC, OC);                                 \
-    i += stride;                                         \
-    base += stride;                                      \
-    if (base >= nv) break;
-    while (base < nv) {
-        GALP_GROUP_STAGE(sa, sb, sc, sd, LA, oA, LB, oB)
-        GALP_GROUP_STAGE(sb, sc, sd, sa, LB, oB, LA, oA)
-        GALP_GROUP_STAGE(sc, sd, sa, sb, LA, oA, LB, oB)
-        GALP_GROUP_STAGE(sd, sa, sb, sc, LB, oB, LA, oA)
-    }
-#undef GALP_GROUP_STAGE
-    block_count_add<4>(changed, counter, scratch);
-}
+struct HashGroupResolve {
+    template <int G>
+    __device__ __forceinline__ static uint32_t best(int32_t L) { return group_best<G, (G < 4 ? G : 4)>(L); }
+};
 
 // Open-addressing insert into an LDS table of 2^BITS slots; returns the slot
 // and sets *created when this lane claimed an empty slot.  The caller
@@ -645,14 +575,13 @@ static void launch_group(galp_graph *g, int bid, const int32_t *prev, int32_t *n
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
-    constexpr int PEEL = G < 4 ? G : 4;
     const int64_t per_block = 256 / G;
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
     if (g->d_vlist)
-        k_hash_group<G, PEEL, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+        k_group<G, HashGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
             (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
     else
-        k_hash_group<G, PEEL, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+        k_group<G, HashGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
             (const E *)g->d_off, g->d_nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
     prof_end(g, &pl);
 }

@@ -1,0 +1,743 @@
+// lp_sort.hip -- method 0 (sort / segmented reduce) on gfx950.
+//
+// Replaces the data-parallel-primitive pipeline of the reference:
+//   DPPBase::perform_lp          label_propagation/src/dpp/dpp.cuh:104-138
+//   gather_labels                src/dpp/kernel.cuh:27-37
+//   SegmentedSorter::run         src/common/segmented_sort.cuh:35-64 (moderngpu)
+//   find_boundaries .. compute_label_weights   src/dpp/kernel.cuh:41-92
+//   SegmentedReducer::apply      src/common/segmented_reduce.cuh:559-626
+//   update_labels                src/dpp/kernel.cuh:95-119
+//   compute_max_labels (li)      src/dpp/kernel.cuh:135-206, incore_li.cuh:43-87
+// Design (not a translation): the reference sorts all m gathered labels with
+// a global segmented merge sort and then runs five m-sized passes (flags,
+// scan, scatter, weights, segmented arg-max).  Here each degree class sorts
+// where its rows fit:
+//   - degree <= 64: a bitonic sort of (label, position) across the group's
+//     lanes (register shuffles), then run ends and run starts from ballots;
+//   - 65..4096: a bitonic sort of (label, position) in LDS per vertex, run
+//     lengths by binary search for each run's first element;
+//   - hubs (> 4096): a segmented LSD radix sort (8-bit digits, as many passes
+//     as the label width needs) of the hub edges in a compact buffer, then a
+//     per-tile run-length pass; tiles reduce (count, last) with one 64-bit
+//     atomicMax per tile per hub.
+// The sort key is (label, position in the row), so every sort is stable in
+// the row order and the last element of a run holds the run's largest
+// position: the winner argmax (count, last) equals the hash method's.
+
+#include <algorithm>
+#include <vector>
+
+#include "device_common.hpp"
+#include "group_kernel.hpp"
+#include "internal.hpp"
+#include "pipeline.hpp"
+
+namespace galp {
+
+// ---------------------------------------------------------------------------
+// Degree <= G.  Bitonic sort (Batcher 1968) of the G keys (label, lane)
+// across the group with lane exchanges; -1 (no edge) sorts last as
+// 0xffffffff.  Then lane p is a run end if the next key differs, a run start
+// if the previous one does; the run's length comes from the highest start
+// at or below p (one ballot), its last position is the sorted lane's
+// original index.
+struct SortGroupResolve {
+    template <int G>
+    __device__ __forceinline__ static uint32_t best(int32_t L) {
+        const int lane = lane_id();
+        const int p = lane & (G - 1);
+        uint32_t key = L >= 0 ? (uint32_t)L : 0xffffffffu;
+        uint32_t pos = (uint32_t)p;
+#pragma unroll
+        for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t ok = (uint32_t)__shfl_xor((int)key, j, G);
+                const uint32_t op = (uint32_t)__shfl_xor((int)pos, j, G);
+                const bool up = (p & k) == 0;
+                const bool lower = (p & j) == 0;
+                const bool other_less = ok < key || (ok == key && op < pos);
+                const bool take = (lower == up) ? other_less : !other_less;
+                key = take ? ok : key;
+                pos = take ? op : pos;
+            }
+        }
+        const uint32_t knext = (uint32_t)__shfl_down((int)key, 1, G);
+        const uint32_t kprev = (uint32_t)__shfl_up((int)key, 1, G);
+        const bool valid = key != 0xffffffffu;
+        const bool is_end = valid && (p == G - 1 || knext != key);
+        const bool is_start = valid && (p == 0 || kprev != key);
+        const unsigned long long starts = __ballot(is_start);
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int s = 63 - __clzll((long long)((starts & upto) | 1ull));   // my run's start (in my group when valid)
+        const uint32_t cnt = (uint32_t)(lane - s + 1);
+        return group_max<G>(is_end ? ((cnt << 8) | pos) : 0u);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// 64 < degree <= T*EPT: one block of T threads per vertex.  The (label,
+// position) keys go to LDS, padded with all-ones to the next power of two,
+// and a bitonic network sorts them; each run end finds its run's first
+// element by binary search (lower bound of its label), giving the count; the
+// block max of (count << 32 | last) picks the winner, whose label the
+// thread that found it writes.
+template <int T, int EPT, typename E>
+struct LdsSortPolicy {
+    static constexpr int N = T * EPT;
+    struct Item {
+        bool valid;
+        int len;
+        int64_t v;
+        int64_t b;
+        int32_t old;
+    };
+    const E *off;
+    const int32_t *prev;
+    int32_t *next;
+    const int32_t *vlist;
+    int64_t v0, nv;
+    unsigned long long *skey;
+    unsigned long long *sred64;
+    int32_t *err;
+    uint32_t changed;
+
+    __device__ Item item(int64_t i) const {
+        Item it{false, 0, 0, 0, 0};
+        if (i < nv) {
+            it.v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
+            it.b = (int64_t)off[it.v];
+            it.len = (int)((int64_t)off[it.v + 1] - it.b);
+            it.old = prev[it.v];
+            it.valid = true;
+            if (it.len > N) {   // never planned into this class
+                if (threadIdx.x == 0) atomicOr(err, 8);
+                it.len = 0;
+            }
+        }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.b; }
+
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        if (it.len == 0) return;
+        const int len = it.len;
+        int n2 = 1;
+        while (n2 < len) n2 <<= 1;
+        const int tid = (int)threadIdx.x;
+        __syncthreads();   // the previous vertex's readers are done
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + tid;
+            if (pos < n2)
+                skey[pos] = pos < len ? (((unsigned long long)(uint32_t)lab[k] << 32) | (uint32_t)pos) : ~0ull;
+        }
+        __syncthreads();
+        for (int kk = 2; kk <= n2; kk <<= 1) {
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int q = tid; q < (n2 >> 1); q += T) {
+                    const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                    const int l = i + j;
+                    const unsigned long long a = skey[i], b = skey[l];
+                    const bool up = (i & kk) == 0;
+                    if ((a > b) == up) {
+                        skey[i] = b;
+                        skey[l] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        unsigned long long best = 0;
+        int32_t bestlab = 0;
+        for (int q = tid; q < len; q += T) {
+            const uint32_t key = (uint32_t)(skey[q] >> 32);
+            const bool end = q == len - 1 || (uint32_t)(skey[q + 1] >> 32) != key;
+            if (end) {
+                int lo = 0, hi = q;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((uint32_t)(skey[mid] >> 32) < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                const unsigned long long val =
+                    ((unsigned long long)(q - lo + 1) << 32) | (uint32_t)skey[q];
+                if (val > best) {
+                    best = val;
+                    bestlab = (int32_t)key;
+                }
+            }
+        }
+        const unsigned long long top = block_max<T / kWave>(best, sred64);
+        if (best == top) {   // unique: positions are distinct
+            next[it.v] = bestlab;
+            changed += (bestlab != it.old);
+        }
+    }
+};
+
+template <int T, int EPT, typename E>
+__global__ void __launch_bounds__(T)
+k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+           int32_t *__restrict__ next, const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+           unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+    __shared__ unsigned long long skey[T * EPT];
+    __shared__ unsigned long long sred64[T / kWave];
+    __shared__ uint32_t sred[T / kWave];
+    if (*stop) return;
+    LdsSortPolicy<T, EPT, E> pol{off, prev, next, vlist, v0, nv, skey, sred64, err, 0u};
+    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    __syncthreads();
+    block_count_add<T / kWave>(pol.changed, counter, sred);
+}
+
+// ---------------------------------------------------------------------------
+// Hubs.  The edges of hub h occupy [hoff[h], hoff[h+1]) of a compact key
+// buffer, in kHubTile-edge tiles (the hub path's tile list: tiles of a hub
+// are consecutive).  Each LSD pass p sorts by bits [8p, 8p+8) of the label:
+//   hist:    per tile, the 256-digit histogram, stored hub-major, then
+//            digit-major, then tile: idx = tile0[h]*256 + d*ntiles[h] + tl;
+//   scan:    one exclusive scan over that array gives, for every (hub,
+//            digit, tile), the tile's first output slot -- hub-major order
+//            keeps every hub inside its own range, so the sort is segmented;
+//   scatter: per tile, a stable multisplit: wave w owns elements
+//            [512w, 512w+512) in order; 8 ballots give each lane the lanes
+//            of its chunk with the same digit, whose popcount below it is
+//            its rank; per-wave running counters in LDS carry the ranks
+//            across chunks.
+// The payload is the element's position in its row.  Pass 0 gathers the
+// labels (prev[nbr[e]]) into the key buffer and takes the position from the
+// index.
+constexpr int kSortTileT = 256;
+constexpr int kSortEPT = kHubTile / kSortTileT;   // 8
+static_assert(kHubTile == 2048 && kSortEPT == 8, "hub tile layout");
+
+struct HubTile {
+    int h;
+    int64_t tl;      // tile index within its hub
+    int64_t c0;      // first compact slot of the tile
+    int len;         // elements in the tile
+    int64_t hbase;   // tile0[h] * 256
+    int64_t nth;     // tiles of the hub
+};
+
+__device__ __forceinline__ HubTile hub_tile_at(int64_t t, const int32_t *tile_hub, const int32_t *hub_tile0,
+                                               const int32_t *hub_ntiles, const int64_t *hoff) {
+    HubTile x;
+    x.h = tile_hub[t];
+    const int64_t t0 = hub_tile0[x.h];
+    x.tl = t - t0;
+    x.nth = hub_ntiles[x.h];
+    x.hbase = t0 * 256;
+    const int64_t hb = hoff[x.h], he = hoff[x.h + 1];
+    x.c0 = hb + x.tl * kHubTile;
+    x.len = (int)min((int64_t)kHubTile, he - x.c0);
+    return x;
+}
+
+// Pass-0 histogram; also gathers the labels into the key buffer.
+template <typename E>
+__global__ void __launch_bounds__(kSortTileT)
+k_hs_gather_hist(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+                 const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ tile_hub,
+                 const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
+                 const int64_t *__restrict__ hoff, int64_t ntiles, uint32_t *__restrict__ keys,
+                 uint32_t *__restrict__ hist, const int32_t *__restrict__ stop) {
+    __shared__ uint32_t sh[256];
+    if (*stop) return;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
+        const int64_t rb = (int64_t)off[hub_vid[x.h]] + x.tl * kHubTile;
+        sh[threadIdx.x] = 0;
+        __syncthreads();
+        int32_t u[kSortEPT];
+#pragma unroll
+        for (int k = 0; k < kSortEPT; ++k) {
+            const int j = k * kSortTileT + (int)threadIdx.x;
+            u[k] = __builtin_nontemporal_load(&nbr[rb + (j < x.len ? j : 0)]);
+        }
+#pragma unroll
+        for (int k = 0; k < kSortEPT; ++k) {
+            const int j = k * kSortTileT + (int)threadIdx.x;
+            const uint32_t L = (uint32_t)prev[u[k]];
+            if (j < x.len) {
+                keys[x.c0 + j] = L;
+                atomicAdd(&sh[L & 0xffu], 1u);
+            }
+        }
+        __syncthreads();
+        hist[x.hbase + (int64_t)threadIdx.x * x.nth + x.tl] = sh[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// Histogram of digit `shift` for passes >= 1.
+__global__ void __launch_bounds__(kSortTileT)
+k_hs_hist(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
+          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t ntiles,
+          const uint32_t *__restrict__ keys, int shift, uint32_t *__restrict__ hist,
+          const int32_t *__restrict__ stop) {
+    __shared__ uint32_t sh[256];
+    if (*stop) return;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
+        sh[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t kv[kSortEPT];
+#pragma unroll
+        for (int k = 0; k < kSortEPT; ++k) {
+            const int j = k * kSortTileT + (int)threadIdx.x;
+            kv[k] = keys[x.c0 + (j < x.len ? j : 0)];
+        }
+#pragma unroll
+        for (int k = 0; k < kSortEPT; ++k) {
+            const int j = k * kSortTileT + (int)threadIdx.x;
+            if (j < x.len) atomicAdd(&sh[(kv[k] >> shift) & 0xffu], 1u);
+        }
+        __syncthreads();
+        hist[x.hbase + (int64_t)threadIdx.x * x.nth + x.tl] = sh[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of n uint32 into int64, in three launches: per-chunk sums,
+// a one-block scan of the sums, per-chunk scans from their bases.
+constexpr int kScanT = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kScanT * kScanPer;   // 4096
+
+template <int NW>
+__device__ __forceinline__ int64_t block_exclusive_sum(int64_t x, int64_t *sw, int64_t *total) {
+    const int lane = lane_id();
+    const int w = (int)(threadIdx.x / kWave);
+    int64_t inc = x;
+#pragma unroll
+    for (int s = 1; s < kWave; s <<= 1) {
+        const int64_t y = __shfl_up(inc, s);
+        if (lane >= s) inc += y;
+    }
+    if (lane == kWave - 1) sw[w] = inc;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        if (i < w) before += sw[i];
+        all += sw[i];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - x;
+}
+
+__global__ void __launch_bounds__(kScanT)
+k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, int64_t *__restrict__ bsum, const int32_t *__restrict__ stop) {
+    __shared__ unsigned long long sred64[kScanT / kWave];
+    if (*stop) return;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    unsigned long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int64_t i = base + (int64_t)k * kScanT + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane_id() == 0) sred64[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kScanT / kWave; ++i) t += sred64[i];
+        bsum[blockIdx.x] = (int64_t)t;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_scan_top(int64_t *__restrict__ bsum, int64_t nb, const int32_t *__restrict__ stop) {
+    __shared__ int64_t sw[1024 / kWave];
+    if (*stop) return;
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t b0 = (int64_t)threadIdx.x * per;
+    int64_t s = 0;
+    for (int64_t i = b0; i < b0 + per && i < nb; ++i) s += bsum[i];
+    int64_t total;
+    int64_t run = block_exclusive_sum<1024 / kWave>(s, sw, &total);
+    for (int64_t i = b0; i < b0 + per && i < nb; ++i) {
+        const int64_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(kScanT)
+k_scan_down(const uint32_t *__restrict__ in, int64_t n, const int64_t *__restrict__ bsum,
+            int64_t *__restrict__ out, const int32_t *__restrict__ stop) {
+    __shared__ int64_t sw[kScanT / kWave];
+    if (*stop) return;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    // thread t scans the kScanPer consecutive entries [base + t*kScanPer, ...)
+    const int64_t i0 = base + (int64_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = i0 + k < n ? in[i0 + k] : 0u;
+        s += v[k];
+    }
+    int64_t total;
+    int64_t run = bsum[blockIdx.x] + block_exclusive_sum<kScanT / kWave>(s, sw, &total);
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        if (i0 + k < n) out[i0 + k] = run;
+        run += v[k];
+    }
+}
+
+// Lanes of this wave whose 8-bit digit equals this lane's, among `valid`.
+__device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool valid) {
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    return peers;
+}
+
+// Stable scatter of one pass.  FIRST: source payload is the position itself.
+template <bool FIRST>
+__global__ void __launch_bounds__(kSortTileT)
+k_hs_scatter(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
+             const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t ntiles,
+             const uint32_t *__restrict__ ksrc, const uint32_t *__restrict__ psrc, int shift,
+             const int64_t *__restrict__ scan, uint32_t *__restrict__ kdst, uint32_t *__restrict__ pdst,
+             const int32_t *__restrict__ stop) {
+    constexpr int NW = kSortTileT / kWave;   // 4 waves, 512 elements each
+    constexpr int CH = kHubTile / NW / kWave;   // 8 chunks of 64 per wave
+    __shared__ uint32_t whist[NW][256];
+    __shared__ int64_t wbase[NW][256];
+    if (*stop) return;
+    const int lane = lane_id();
+    const int w = (int)(threadIdx.x / kWave);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
+        for (int i = threadIdx.x; i < NW * 256; i += kSortTileT) (&whist[0][0])[i] = 0;
+        uint32_t kv[CH], pv[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int j = w * (CH * kWave) + c * kWave + lane;
+            const int jc = j < x.len ? j : 0;
+            kv[c] = ksrc[x.c0 + jc];
+            pv[c] = FIRST ? (uint32_t)(x.tl * kHubTile + j) : psrc[x.c0 + jc];
+        }
+        __syncthreads();
+        unsigned long long peers[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int j = w * (CH * kWave) + c * kWave + lane;
+            const bool valid = j < x.len;
+            const uint32_t d = (kv[c] >> shift) & 0xffu;
+            peers[c] = digit_peers(d, valid);
+            if (valid && (peers[c] & lt) == 0) whist[w][d] += (uint32_t)__popcll(peers[c]);   // lowest peer
+        }
+        __syncthreads();
+        {
+            const int d = (int)threadIdx.x;   // 256 threads = 256 digits
+            int64_t b = scan[x.hbase + (int64_t)d * x.nth + x.tl];
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) {
+                wbase[ww][d] = b;
+                b += whist[ww][d];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int j = w * (CH * kWave) + c * kWave + lane;
+            const bool valid = j < x.len;
+            const uint32_t d = (kv[c] >> shift) & 0xffu;
+            const int64_t base = wbase[w][d];
+            if (valid) {
+                const int64_t dst = base + __popcll(peers[c] & lt);
+                kdst[dst] = kv[c];
+                pdst[dst] = pv[c];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (valid && (peers[c] & lt) == 0) wbase[w][d] = base + __popcll(peers[c]);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+    }
+}
+
+// Run lengths of the sorted hub rows.  Thread t of a tile owns the
+// kSortEPT consecutive slots [t*8, t*8+8); a block max-scan of run starts
+// gives each run end its start; a run that began in an earlier tile finds
+// its start by binary search over the hub's sorted keys.  One atomicMax of
+// the tile's best (count << 32 | last) per tile.
+__global__ void __launch_bounds__(kSortTileT)
+k_hs_runs(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
+          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t ntiles,
+          const uint32_t *__restrict__ keys, const uint32_t *__restrict__ pos,
+          unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop) {
+    constexpr int NW = kSortTileT / kWave;
+    __shared__ int64_t sw[NW];
+    __shared__ unsigned long long sred64[NW];
+    if (*stop) return;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
+        const int64_t hb = hoff[x.h], he = hoff[x.h + 1];
+        const int j0 = (int)threadIdx.x * kSortEPT;
+        uint32_t kv[kSortEPT + 1];
+        int64_t last_start = -1;   // hub-local index of the last run start in my slots
+#pragma unroll
+        for (int k = 0; k <= kSortEPT; ++k) {
+            const int64_t g = x.c0 + j0 + k;
+            kv[k] = g < he ? keys[g] : 0xffffffffu;
+        }
+        const uint32_t kbefore = (x.c0 + j0 > hb) ? keys[x.c0 + j0 - 1] : 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < kSortEPT; ++k) {
+            const int j = j0 + k;
+            const uint32_t kp = k == 0 ? kbefore : kv[k - 1];
+            const bool start = j < x.len && (x.c0 + j == hb || kp != kv[k]);
+            if (start) last_start = x.c0 + j - hb;
+        }
+        // exclusive max-scan of last_start over threads (wave scan + waves)
+        int64_t inc = last_start;
+        const int lane = lane_id();
+        const int w = (int)(threadIdx.x / kWave);
+#pragma unroll
+        for (int s = 1; s < kWave; s <<= 1) {
+            const int64_t y = __shfl_up(inc, s);
+            if (lane >= s) inc = inc > y ? inc : y;
+        }
+        if (lane == kWave - 1) sw[w] = inc;
+        __syncthreads();
+        int64_t before = -1;
+#pragma unroll
+        for (int i = 0; i < NW; ++i)
+            if (i < w) before = before > sw[i] ? before : sw[i];
+        __syncthreads();
+        int64_t ex = __shfl_up(inc, 1);
+        if (lane == 0) ex = -1;
+        int64_t cur = ex > before ? ex : before;   // start of the run open at my first slot
+        unsigned long long best = 0;
+#pragma unroll
+        for (int k = 0; k < kSortEPT; ++k) {
+            const int j = j0 + k;
+            if (j >= x.len) break;
+            const uint32_t kp = k == 0 ? kbefore : kv[k - 1];
+            const int64_t gl = x.c0 + j - hb;   // hub-local index
+            if (gl == 0 || kp != kv[k]) cur = gl;
+            const bool end = (x.c0 + j + 1 == he) || kv[k + 1] != kv[k];
+            if (end) {
+                int64_t st = cur;
+                if (st < 0) {   // the run began in an earlier tile
+                    int64_t lo = 0, hi = x.c0 - hb;
+                    while (lo < hi) {
+                        const int64_t mid = (lo + hi) >> 1;
+                        if (keys[hb + mid] < kv[k]) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    st = lo;
+                }
+                const unsigned long long val = ((unsigned long long)(gl - st + 1) << 32) | pos[x.c0 + j];
+                best = val > best ? val : best;
+            }
+        }
+        best = block_max<NW>(best, sred64);
+        if (threadIdx.x == 0 && best) atomicMax(&hub_best[x.h], best);
+        __syncthreads();
+    }
+}
+
+// Hub winners: the label at position `last` of the row; resets hub_best.
+template <typename E>
+__global__ void __launch_bounds__(256)
+k_hs_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ hub_vid,
+           int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ prev,
+           int32_t *__restrict__ next, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
+    __shared__ uint32_t sred[4];
+    if (*stop) return;
+    uint32_t changed = 0;
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < nhubs; base += (int64_t)gridDim.x * 256) {
+        const int64_t h = base + threadIdx.x;
+        if (h < nhubs) {
+            const int64_t v = hub_vid[h];
+            const unsigned long long b = hub_best[h];
+            hub_best[h] = 0ull;
+            const int32_t w = prev[nbr[(int64_t)off[v] + (uint32_t)b]];
+            next[v] = w;
+            changed += (w != prev[v]);
+        }
+    }
+    block_count_add<4>(changed, counter, sred);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static int grid_for(const galp_graph *g, int64_t work_items, int blocks_per_cu) {
+    const int64_t cap = (int64_t)g->ctx->prop.multiProcessorCount * blocks_per_cu;
+    int64_t b = work_items < cap ? work_items : cap;
+    return (int)(b < 1 ? 1 : b);
+}
+
+template <class T>
+static int salloc(galp_graph *g, T **p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    if (hipMalloc((void **)p, sizeof(T) * (size_t)count) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(g->ctx, GALP_ENOMEM, "sort-method allocation failed");
+    }
+    g->device_bytes += (int64_t)(sizeof(T) * (size_t)count);
+    return GALP_OK;
+}
+
+void sort_free(galp_graph *g) {
+    SortState *S = g->sort;
+    if (!S) return;
+    void *ptrs[] = {S->d_hoff, S->d_kA, S->d_kB, S->d_pA, S->d_pB, S->d_hist, S->d_scan, S->d_bsum};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete S;
+    g->sort = nullptr;
+}
+
+int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg) {
+    if (g->sort) return GALP_OK;
+    SortState *S = new (std::nothrow) SortState();
+    if (!S) return fail(g->ctx, GALP_ENOMEM, "host allocation failed");
+    g->sort = S;
+    const int bits = g->n > 1 ? 64 - __builtin_clzll((unsigned long long)(g->n - 1)) : 1;
+    S->passes = (bits + 7) / 8;
+    std::vector<int64_t> hoff(hub_deg.size() + 1, 0);
+    for (size_t h = 0; h < hub_deg.size(); ++h) hoff[h + 1] = hoff[h] + hub_deg[h];
+    S->eh = hoff.back();
+    if (!g->nhubs) return GALP_OK;
+    S->nscan = g->nhub_tiles * 256;
+    S->nbsum = (S->nscan + kScanChunk - 1) / kScanChunk;
+    int rc;
+    if ((rc = salloc(g, &S->d_hoff, (int64_t)hoff.size())) || (rc = salloc(g, &S->d_kA, S->eh)) ||
+        (rc = salloc(g, &S->d_kB, S->eh)) || (rc = salloc(g, &S->d_pA, S->eh)) || (rc = salloc(g, &S->d_pB, S->eh)) ||
+        (rc = salloc(g, &S->d_hist, S->nscan)) || (rc = salloc(g, &S->d_scan, S->nscan)) ||
+        (rc = salloc(g, &S->d_bsum, S->nbsum)))
+        return rc;
+    if (hipMemcpy(S->d_hoff, hoff.data(), 8 * hoff.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "hub offsets upload failed");
+    return GALP_OK;
+}
+
+template <typename E>
+static double class_bytes(const BucketRange &r) {
+    // offsets, neighbour ids, gathered labels, own previous label, next label
+    return (double)sizeof(E) * (double)(r.nv + 1) + 8.0 * (double)r.edges + 8.0 * (double)r.nv;
+}
+
+template <int G, typename E>
+static void sort_group(galp_graph *g, int bid, const int32_t *prev, int32_t *next) {
+    const BucketRange &r = g->bucket[bid];
+    if (!r.nv) return;
+    PendingLaunch pl;
+    prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
+    const int64_t per_block = 256 / G;
+    const int64_t blocks = (r.nv + per_block - 1) / per_block;
+    if (g->d_vlist)
+        k_group<G, SortGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
+    else
+        k_group<G, SortGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+            (const E *)g->d_off, g->d_nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
+    prof_end(g, &pl);
+}
+
+template <int T, int EPT, typename E>
+static void sort_lds(galp_graph *g, int bid, const int32_t *prev, int32_t *next, int blocks_per_cu) {
+    const BucketRange &r = g->bucket[bid];
+    if (!r.nv) return;
+    PendingLaunch pl;
+    prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
+    k_sort_lds<T, EPT, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+        (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
+    prof_end(g, &pl);
+}
+
+template <typename E>
+static int sort_hubs(galp_graph *g, const int32_t *prev, int32_t *next) {
+    SortState *S = g->sort;
+    const int64_t nt = g->nhub_tiles;
+    const int gt = grid_for(g, nt, 8);
+    const int nbs = (int)S->nbsum;
+    PendingLaunch pl;
+    prof_begin(g, "hub_sort", 8.0 * (double)g->bucket[B_HUB].edges, &pl);
+    k_hs_gather_hist<E><<<gt, kSortTileT, 0, g->stream>>>((const E *)g->d_off, g->d_nbr, prev, g->d_hub_vid,
+                                                          g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff,
+                                                          nt, S->d_kA, S->d_hist, g->d_stop);
+    uint32_t *ks = S->d_kA, *ps = S->d_pA, *kd = S->d_kB, *pd = S->d_pB;
+    for (int p = 0; p < S->passes; ++p) {
+        if (p > 0)
+            k_hs_hist<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, nt,
+                                                        ks, 8 * p, S->d_hist, g->d_stop);
+        k_scan_reduce<<<nbs, kScanT, 0, g->stream>>>(S->d_hist, S->nscan, S->d_bsum, g->d_stop);
+        k_scan_top<<<1, 1024, 0, g->stream>>>(S->d_bsum, S->nbsum, g->d_stop);
+        k_scan_down<<<nbs, kScanT, 0, g->stream>>>(S->d_hist, S->nscan, S->d_bsum, S->d_scan, g->d_stop);
+        if (p == 0)
+            k_hs_scatter<true><<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles,
+                                                                 S->d_hoff, nt, ks, nullptr, 0, S->d_scan, kd, pd,
+                                                                 g->d_stop);
+        else
+            k_hs_scatter<false><<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles,
+                                                                  S->d_hoff, nt, ks, ps, 8 * p, S->d_scan, kd, pd,
+                                                                  g->d_stop);
+        std::swap(ks, kd);
+        std::swap(ps, pd);
+    }
+    k_hs_runs<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, nt, ks, ps,
+                                                 g->d_hub_best, g->d_stop);
+    k_hs_final<E><<<grid_for(g, (g->nhubs + 255) / 256, 4), 256, 0, g->stream>>>(
+        (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, prev, next, g->d_counter, g->d_stop);
+    prof_end(g, &pl);
+    return GALP_OK;
+}
+
+template <typename E>
+static int sort_iteration(galp_graph *g) {
+    const int32_t *prev = g->d_lab[g->cur];
+    int32_t *next = g->d_lab[g->cur ^ 1];
+    if (g->nhubs) {
+        const int rc = sort_hubs<E>(g, prev, next);
+        if (rc != GALP_OK) return rc;
+    }
+    if (g->opts.policy == GALP_POLICY_LI) {
+        // "load-imbalanced" ablation: every class <= 4096 through the
+        // 512-thread block-per-vertex kernel, as the reference's li path
+        // runs one block per vertex whatever its degree.
+        for (int b = B_B4K; b >= B_G4; --b) sort_lds<512, 8, E>(g, b, prev, next, 2);
+    } else {
+        sort_lds<512, 8, E>(g, B_B4K, prev, next, 2);
+        sort_lds<256, 8, E>(g, B_B2K, prev, next, 4);
+        sort_lds<128, 8, E>(g, B_B1K, prev, next, 8);
+        sort_lds<64, 8, E>(g, B_W512, prev, next, 16);
+        sort_lds<64, 4, E>(g, B_W256, prev, next, 24);
+        sort_lds<64, 2, E>(g, B_W128, prev, next, 32);
+        sort_group<64, E>(g, B_G64, prev, next);
+        sort_group<32, E>(g, B_G32, prev, next);
+        sort_group<16, E>(g, B_G16, prev, next);
+        sort_group<8, E>(g, B_G8, prev, next);
+        sort_group<4, E>(g, B_G4, prev, next);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("sort kernel launch: ") + hipGetErrorString(e));
+    return GALP_OK;
+}
+
+int launch_sort_iteration(galp_graph *g) {
+    if (!g->sort) return fail(g->ctx, GALP_ESTATE, "internal: sort state missing");
+    return g->off64 ? sort_iteration<int64_t>(g) : sort_iteration<int32_t>(g);
+}
+
+}  // namespace galp

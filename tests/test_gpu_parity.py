@@ -18,7 +18,7 @@ from galp_amd import _abi
 
 pytestmark = pytest.mark.gpu
 
-METHODS = [_abi.METHOD_HASH]
+METHODS = [_abi.METHOD_HASH, _abi.METHOD_SORT]
 REORDER = [1, 0]
 
 
@@ -163,10 +163,20 @@ def test_device_loop_equals_steps(ctx, method):
     assert np.array_equal(labels, final)
 
 
-def test_empty_and_edgeless(ctx):
+def test_sort_li_policy(ctx):
+    """Policy 4 (li): method 0 with every class up to 4096 in the
+    block-per-vertex kernel gives the same labels."""
+    off, nbr = hub_graph(2)
+    check_per_iteration(ctx, off, nbr, 4, "li-hubs", method=_abi.METHOD_SORT, policy=_abi.POLICY_LI)
+    off, nbr = _abi.gen_rmat(14, 16, seed=5)
+    check_per_iteration(ctx, off, nbr, 4, "li-rmat14", method=_abi.METHOD_SORT, policy=_abi.POLICY_LI, reorder=0)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_empty_and_edgeless(ctx, method):
     for off, nbr in ((np.zeros(1, np.int64), np.zeros(0, np.int32)),
                      (np.zeros(6, np.int64), np.zeros(0, np.int32))):
-        with _abi.Graph(ctx, off, nbr) as g:
+        with _abi.Graph(ctx, off, nbr, method=method) as g:
             labels, res = g.run(10)
             assert labels.tolist() == list(range(len(off) - 1))
             assert res["iterations"] == 1   # 0 changes <= n*1e-5 stops at once
@@ -181,7 +191,7 @@ def test_errors(ctx):
         _abi.Graph(ctx, np.array([0, 2, 1], np.int64), np.array([1, 0], np.int32))
     assert e.value.code == -1
     ok_nbr = np.array([1, 0], np.int32)
-    for policy, method in ((3, 1), (4, 1), (5, 1), (3, 0)):
+    for policy, method in ((3, 1), (4, 1), (5, 1), (3, 0), (5, 0)):
         with pytest.raises(_abi.GalpError) as e:
             _abi.Graph(ctx, off, ok_nbr, policy=policy, method=method)
         assert e.value.code == -4
