@@ -7,11 +7,23 @@ m = 128,302,318 directed edges.  One step = one full LP run of 10
 synchronous iterations from labels[i] = i (the reference's timed main loop,
 lfht.cuh:87-106), inputs resident in HBM.  Prints one JSON line.
 
-value      = m * iterations / wall time of the timed steps, in GTEPS
-roofline   = the dominant kernel's algorithmic bytes / its average launch
-             time (HIP events on the launch stream), against 8 TB/s
-cpu_baseline = the CPU oracle (oracle/lp_oracle.c, OpenMP) on the same
-             graph and iterations; its final labels are also the parity check
+Variants, each timed over the same steps:
+  hash_learned  method 1, learned dispatch: the 4-32-32-1 network is trained
+                on the GPU from distinct-label counts logged during one
+                untimed plain run of the same graph (training and logging
+                are outside the timed region; inference is inside)
+  hash_plain    method 1, degree-class dispatch
+  sort          method 0 (segmented sorts), degree-class dispatch
+value        = GTEPS of hash_learned = m * iterations / wall time of the timed
+               steps (host clock around the steps, device synchronised, no
+               instrumentation)
+roofline     = the dominant kernel of hash_learned: its algorithmic bytes per
+               launch / its average launch time, from HIP events on the launch
+               stream over a second, instrumented pass of the same steps,
+               against 8 TB/s
+cpu_baseline = the CPU oracle (oracle/lp_oracle.c, OpenMP) on the same graph
+               and iterations; its final labels are the parity check of every
+               variant
 """
 import argparse
 import json
@@ -24,31 +36,60 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS per LP iteration + % HBM peak; learned vs plain dispatch, bit-exact labels"
+MLP_HIDDEN = 32
+CAND_THRESH = 6.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_variant(ctx, off, nbr, steps, warmup, niter, **opts):
+def train_network(ctx, off, nbr, niter):
+    """Logs (features, distinct-count) samples during one plain run and
+    trains the dispatcher's network on them, all on the GPU."""
+    import numpy as np
     from galp_amd import _abi
-    with _abi.Graph(ctx, off, nbr, profile=1, **opts) as g:
-        for _ in range(warmup):
-            g.run(niter, want_labels=False)
-        g.profile_reset()
-        iters = 0
-        dev_ms = 0.0
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            _, res = g.run(niter, want_labels=False)
-            iters += res["iterations"]
-            dev_ms += res["device_ms"]
-        wall = time.perf_counter() - t0
-        prof = g.profile()
-        labels, _ = g.run(niter)
-        info = g.info()
-    return {"wall_s": wall, "iters": iters, "device_ms": dev_ms, "profile": prof, "labels": labels,
-            "info": info}
+    t = time.perf_counter()
+    with _abi.Graph(ctx, off, nbr) as g:
+        big = int((np.diff(off) > 64).sum())
+        g.log_enable((niter + 1) * max(big, 1))
+        g.run(niter, want_labels=False)
+        x, y = g.log_read()
+    p0 = _abi.mlp_init(MLP_HIDDEN, 1)
+    p, loss = _abi.mlp_train(ctx, MLP_HIDDEN, p0, x, y, epochs=8, batch=512, lr=3e-3, seed=2)
+    log(f"bench: network trained on {len(x)} samples, loss {loss:.4f}, {time.perf_counter() - t:.1f}s")
+    return p, len(x), loss
+
+
+def time_variant(ctx, off, nbr, steps, warmup, niter, mlp=None, **opts):
+    """Times `steps` runs without instrumentation (the value), then the same
+    runs again with a HIP event pair around every kernel launch (the
+    per-kernel roofline; events cost a few percent of wall time)."""
+    from galp_amd import _abi
+    out = {}
+    for profile in (0, 1):
+        with _abi.Graph(ctx, off, nbr, profile=profile, **opts) as g:
+            if mlp is not None:
+                g.set_mlp(MLP_HIDDEN, mlp, CAND_THRESH)
+            for _ in range(warmup):
+                g.run(niter, want_labels=False)
+            g.profile_reset()
+            iters = 0
+            dev_ms = 0.0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                _, res = g.run(niter, want_labels=False)
+                iters += res["iterations"]
+                dev_ms += res["device_ms"]
+            wall = time.perf_counter() - t0
+            if profile:
+                out["profile"] = g.profile()
+                out["profiled_wall_s"] = wall
+            else:
+                out.update({"wall_s": wall, "iters": iters, "device_ms": dev_ms})
+                out["labels"], _ = g.run(niter)
+                out["info"] = g.info()
+    return out
 
 
 def main():
@@ -58,6 +99,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--variants", default="hash_learned,hash_plain,sort")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -75,9 +117,20 @@ def main():
 
     ctx = _abi.Context(0)
     dev = ctx.device_info()
+    names = [v for v in args.variants.split(",") if v]
+    mlp = None
+    train_info = None
+    if "hash_learned" in names:
+        mlp, nsamples, loss = train_network(ctx, off, nbr, args.iters)
+        train_info = {"samples": nsamples, "loss": round(loss, 5), "hidden": MLP_HIDDEN,
+                      "cand_thresh": CAND_THRESH}
+    table = {"hash_learned": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_LEARNED),
+             "hash_plain": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_PLAIN),
+             "sort": dict(method=_abi.METHOD_SORT, dispatch=_abi.DISPATCH_PLAIN)}
     variants = {}
-    for name, opts in (("hash_plain", dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_PLAIN)),):
-        r = time_variant(ctx, off, nbr, args.steps, args.warmup, args.iters, **opts)
+    for name in names:
+        r = time_variant(ctx, off, nbr, args.steps, args.warmup, args.iters,
+                         mlp=mlp if name == "hash_learned" else None, **table[name])
         gteps = m * r["iters"] / r["wall_s"] / 1e9
         variants[name] = r
         log(f"bench: {name}: {r['iters']} iterations in {r['wall_s'] * 1e3:.2f} ms wall "
@@ -87,17 +140,19 @@ def main():
                 log(f"    {k['name']:18s} {k['launches']:5d} launches {k['total_ms'] / k['launches'] * 1e3:9.1f} us avg "
                     f"{k['alg_bytes'] / (k['total_ms'] * 1e-3) / 1e9 if k['total_ms'] > 0 else 0:8.1f} GB/s alg")
 
-    main_v = variants["hash_plain"]
+    head = names[0]
+    main_v = variants[head]
     wall = main_v["wall_s"]
     iters = main_v["iters"]
     gteps = m * iters / wall / 1e9
     per_iter_s = wall / iters
     b_iter = 8.0 * m + 12.0 * n + 4.0
-    prof = [k for k in main_v["profile"] if k["launches"]]
+    prof = [k for k in main_v["profile"] if k["launches"] and k["alg_bytes"] > 0]
     dom = max(prof, key=lambda k: k["total_ms"])
     dom_avg_s = dom["total_ms"] / dom["launches"] * 1e-3
     dom_bytes = dom["alg_bytes"] / dom["launches"]
     achieved = dom_bytes / dom_avg_s / 1e9
+    vg = {k: round(m * v["iters"] / v["wall_s"] / 1e9, 3) for k, v in variants.items()}
 
     out = {
         "metric": METRIC,
@@ -114,7 +169,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"RMAT scale {args.scale}, edge factor 16, a=0.57 b=0.19 c=0.19, seed {args.scale}, "
                                f"{args.iters} synchronous LP iterations per step, in-core",
-                   "n": n, "m": m, "method": "hash (method 1)", "dispatch": "plain",
+                   "n": n, "m": m, "method": "hash (method 1)" if head != "sort" else "sort (method 0)",
+                   "dispatch": "learned" if head == "hash_learned" else "plain",
                    "iterations_per_step": iters // args.steps},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -122,10 +178,13 @@ def main():
                      "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "iteration_roofline": {"alg_bytes": b_iter, "achieved_GBps": round(b_iter / per_iter_s / 1e9, 1),
                                "frac": round(b_iter / per_iter_s / 1e9 / HBM_PEAK_GBS, 4)},
-        "variants": {k: round(m * v["iters"] / v["wall_s"] / 1e9, 3) for k, v in variants.items()},
+        "variants": vg,
         "device": {"name": dev["name"], "arch": dev["arch"], "cus": dev["compute_units"]},
         "preprocess_ms": round(main_v["info"]["preprocess_ms"], 1),
     }
+    if "hash_learned" in vg and "hash_plain" in vg:
+        out["learned_vs_plain"] = round(vg["hash_learned"] / vg["hash_plain"], 3)
+        out["learned_network"] = train_info
 
     if not args.no_cpu:
         import oracle

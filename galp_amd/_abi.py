@@ -23,6 +23,8 @@ STATUS_NAMES = {0: "GALP_OK", -1: "GALP_EINVAL", -2: "GALP_ENOMEM", -3: "GALP_EH
 METHOD_SORT, METHOD_HASH = 0, 1
 POLICY_INCORE, POLICY_OOC_SYNC, POLICY_OOC_ASYNC, POLICY_LI = 0, 1, 2, 4
 DISPATCH_PLAIN, DISPATCH_LEARNED = 0, 1
+# galp_graph_dispatch_stats slots
+DS_CAND_TILES, DS_OVERFLOW, DS_CAND_VERTS, DS_PLAIN_HUBS = 12, 16, 17, 18
 
 
 class GalpError(RuntimeError):

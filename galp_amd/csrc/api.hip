@@ -63,9 +63,15 @@ void prof_begin(galp_graph *g, const char *name, double alg_bytes, PendingLaunch
     };
     pl->agg = g->prof.find(name);
     pl->alg_bytes = alg_bytes;
+    pl->dev_bytes = nullptr;
     pl->e0 = take();
     pl->e1 = take();
     if (pl->e0) (void)hipEventRecord(pl->e0, g->stream);
+}
+
+void prof_begin_dev(galp_graph *g, const char *name, const int64_t *dev_bytes, PendingLaunch *pl) {
+    prof_begin(g, name, 0.0, pl);
+    if (g->prof.on) pl->dev_bytes = dev_bytes;
 }
 
 void prof_end(galp_graph *g, PendingLaunch *pl) {
@@ -83,6 +89,9 @@ static void prof_collect(galp_graph *g) {
             a.launches += 1;
             a.total_ms += ms;
             a.alg_bytes += pl.alg_bytes;
+            int64_t db = 0;
+            if (pl.dev_bytes && hipMemcpy(&db, pl.dev_bytes, 8, hipMemcpyDeviceToHost) == hipSuccess)
+                a.alg_bytes += (double)db;
         }
         if (pl.e0) g->prof.pool.push_back(pl.e0);
         if (pl.e1) g->prof.pool.push_back(pl.e1);
@@ -265,15 +274,18 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     // big vertices (classes W128..HUB) for the learned dispatcher, hubs first
     g->big_vid.clear();
     g->big_deg.clear();
+    g->big_off.clear();
     for (int32_t x : hub_vertices) {
         g->big_vid.push_back(x);
         g->big_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
+        g->big_off.push_back(offi[(size_t)x]);
     }
     for (int b = B_W128; b < B_HUB; ++b)
         for (int64_t k = 0; k < g->bucket[b].nv; ++k) {
             const int32_t x = g->reorder ? (int32_t)(g->bucket[b].v0 + k) : vlist[(size_t)(g->bucket[b].v0 + k)];
             g->big_vid.push_back(x);
             g->big_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
+            g->big_off.push_back(offi[(size_t)x]);
         }
 
     // hubs: phase-1 tiles and phase-2 (hub, label partition) items

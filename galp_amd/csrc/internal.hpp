@@ -65,6 +65,7 @@ struct PendingLaunch {
     int agg;                  // index into Profiler::aggs
     hipEvent_t e0, e1;
     double alg_bytes;
+    const int64_t *dev_bytes; // if set: algorithmic bytes counted on the device
 };
 
 struct Profiler {
@@ -74,6 +75,19 @@ struct Profiler {
     std::vector<hipEvent_t> pool;
     int find(const char *name);
 };
+
+// Candidate tiles come in length classes (<= 128, 256, 512, kCandTile
+// edges), each walked by a kernel that loads just enough slots per lane.
+constexpr int kCandClasses = 4;
+__host__ __device__ constexpr int cand_class(int64_t len) {
+    return len <= 128 ? 0 : len <= 256 ? 1 : len <= 512 ? 2 : 3;
+}
+constexpr int kCtrCandTiles = NUM_BUCKETS;                 // kCandClasses entries
+constexpr int kCtrOverflow = NUM_BUCKETS + kCandClasses;
+constexpr int kCtrCandVerts = kCtrOverflow + 1;
+constexpr int kCtrPlainHubs = kCtrOverflow + 2;           // hubs left to the plain hub path
+constexpr int kCtrCandMt = kCtrOverflow + 3;              // multi-tile vertices routed to candidates
+constexpr int kNumCtrs = kCtrOverflow + 4;
 
 // Candidate path of the learned dispatcher: a wave walks a tile of up to
 // kCandTile edges keeping up to kCandK labels in registers.
@@ -96,6 +110,8 @@ struct LearnState {
     float cand_thresh = 4.0f;          // predicted distinct labels routed to candidates
     // candidate work list (tiles)
     int64_t cw_cap = 0;
+    int64_t cw_base[kCandClasses + 1] = {};   // tile-class segments of the work list
+    int64_t *d_cw_base = nullptr;
     int32_t *d_cw_v = nullptr, *d_cw_len = nullptr, *d_cw_pslot = nullptr;
     int64_t *d_cw_e0 = nullptr;
     // multi-tile vertices and their partial candidates
@@ -103,6 +119,9 @@ struct LearnState {
     int32_t *d_mt_vid = nullptr, *d_mt_of = nullptr;        // mt -> vertex, vertex -> mt or -1
     int32_t *d_mt_ctile0 = nullptr, *d_mt_nctiles = nullptr;
     int32_t *d_mt_cand = nullptr;      // mt vertex routed to candidates this iteration
+    // fixed tile list of the mt vertices: vertex, first edge, length, partial slot, mt index
+    int32_t *d_mt_tv = nullptr, *d_mt_tlen = nullptr, *d_mt_tslot = nullptr, *d_mt_tmt = nullptr;
+    int64_t *d_mt_te0 = nullptr;
     int32_t *d_pn = nullptr;           // candidates per tile, -1 on overflow
     uint32_t *d_pkey = nullptr, *d_pcnt = nullptr, *d_plast = nullptr;
     int32_t *d_hub_sel = nullptr;      // hub left to the plain hub path this iteration
@@ -113,6 +132,7 @@ struct LearnState {
     // device counters: [0..NUM_BUCKETS) list sizes, then candidate tiles,
     // candidate overflows, vertices routed to candidates
     int64_t *d_plist_n = nullptr;
+    int64_t *d_cand_bytes = nullptr;   // algorithmic bytes of the candidate path, per iteration slot
     // training-sample log: x[cap][4], y[cap]
     int64_t log_cap = 0;
     float *d_log_x = nullptr, *d_log_y = nullptr;
@@ -132,10 +152,7 @@ struct SortState {
     int64_t nscan = 0, nbsum = 0;
 };
 
-constexpr int kCtrCandTiles = NUM_BUCKETS;
-constexpr int kCtrOverflow = NUM_BUCKETS + 1;
-constexpr int kCtrCandVerts = NUM_BUCKETS + 2;
-constexpr int kNumCtrs = NUM_BUCKETS + 3;
+
 
 }  // namespace galp
 
@@ -196,6 +213,7 @@ struct galp_graph {
     galp::SortState *sort = nullptr;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
     std::vector<int64_t> big_deg;
+    std::vector<int64_t> big_off;      // row start of each big vertex
 };
 
 namespace galp {
@@ -233,5 +251,8 @@ int dev_unmap_labels(galp_graph *g, const int32_t *in, int32_t *lab);
 // Profiling helpers (api.hip): bracket a launch with events when enabled.
 void prof_begin(galp_graph *g, const char *name, double alg_bytes, PendingLaunch *pl);
 void prof_end(galp_graph *g, PendingLaunch *pl);
+// As prof_begin, with the launch's algorithmic bytes read from the device
+// when the timings are collected.
+void prof_begin_dev(galp_graph *g, const char *name, const int64_t *dev_bytes, PendingLaunch *pl);
 
 }  // namespace galp

@@ -19,6 +19,7 @@
 // (lfht.cuh:127-165).
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -88,31 +89,45 @@ __global__ void k_reset_ctrs(int64_t *ctr) {
     if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
 }
 
+__global__ void k_reset_front(int64_t *ctr, int64_t *cand_bytes) {
+    if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *cand_bytes = 0;
+}
+
 // One block routes kDispChunk consecutive big vertices.  Appends are
 // aggregated per block: the block counts its entries per list in LDS, takes
 // one global atomicAdd per list, then hands out positions from LDS cursors.
 // (Same-address global atomics serialise at the L2; one per wave and list
 // cost ~100 us per iteration on RMAT-22, one per block and list does not.)
-constexpr int kDispChunk = 1024;
-enum : uint8_t { kCodeCand = NUM_BUCKETS, kCodeCandMt, kCodeHub, kCodeNone };
+constexpr int kDispChunk = 256;   // one vertex per thread: the MLP is LDS-latency bound
+enum : uint8_t { kCodeCand = NUM_BUCKETS, kCodeCandMt, kCodeHub };
+// LDS list slots: [0, NUM_BUCKETS) class lists, then kCandClasses tile
+// classes, candidate vertices, plain hubs
+constexpr int kSlotTiles = NUM_BUCKETS;
+constexpr int kSlotCandV = NUM_BUCKETS + kCandClasses;
+constexpr int kSlotHubs = kSlotCandV + 1;
+constexpr int kSlotMt = kSlotHubs + 1;
+constexpr int kSlots = kSlotMt + 1;
 
 template <int H, typename E>
 __global__ void __launch_bounds__(256)
 k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t big_v0, int64_t big_n,
            const float *__restrict__ params, const float *__restrict__ fdeg, const float *__restrict__ fmean,
            const float *__restrict__ fmax, const int32_t *__restrict__ dprev, float cand_thresh, int64_t nhubs,
-           const int32_t *__restrict__ mt_of, const int32_t *__restrict__ mt_ctile0,
-           int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel, int32_t *__restrict__ cw_v,
-           int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len, int32_t *__restrict__ cw_pslot,
-           int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr, const int32_t *__restrict__ stop) {
+           const int32_t *__restrict__ mt_of, int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel,
+           int32_t *__restrict__ cw_v, int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len,
+           int32_t *__restrict__ cw_pslot, const int64_t *__restrict__ cw_base, int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr,
+           int64_t *__restrict__ cand_bytes, const int32_t *__restrict__ stop) {
     __shared__ float sp[mlp_num_params(H)];
     __shared__ uint8_t scode[kDispChunk];
-    __shared__ int32_t sv[kDispChunk], smt[kDispChunk];
+    __shared__ int32_t sv[kDispChunk];
     __shared__ int64_t sb[kDispChunk], sd[kDispChunk];
-    __shared__ unsigned long long scount[NUM_BUCKETS + 2];   // class lists, candidate tiles, candidate vertices
+    __shared__ unsigned long long scount[kSlots];
+    __shared__ unsigned long long sbytes;
     if (*stop) return;
     for (int i = threadIdx.x; i < mlp_num_params(H); i += 256) sp[i] = params[i];
-    if (threadIdx.x < NUM_BUCKETS + 2) scount[threadIdx.x] = 0;
+    if (threadIdx.x < kSlots) scount[threadIdx.x] = 0;
+    if (threadIdx.x == 0) sbytes = 0;
     __syncthreads();
     const int64_t i0 = (int64_t)blockIdx.x * kDispChunk;
     const int nloc = (int)min((int64_t)kDispChunk, big_n - i0);
@@ -131,31 +146,42 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
         uint8_t code;
         if (mt >= 0) {
             mt_cand[mt] = cand ? 1 : 0;
-            if (mt < nhubs) hub_sel[mt] = cand ? 0 : 1;
+            if (mt < nhubs) {
+                hub_sel[mt] = cand ? 0 : 1;
+                if (!cand) atomicAdd(&scount[kSlotHubs], 1ull);
+            }
             code = cand ? (uint8_t)kCodeCandMt : cls < B_HUB ? (uint8_t)cls : (uint8_t)kCodeHub;
         } else {
             code = cand ? (uint8_t)kCodeCand : (uint8_t)cls;
         }
         if (code < NUM_BUCKETS) atomicAdd(&scount[code], 1ull);
-        if (code == kCodeCand) atomicAdd(&scount[NUM_BUCKETS], 1ull);
-        if (code == kCodeCandMt) atomicAdd(&scount[NUM_BUCKETS], (unsigned long long)((d + kCandTile - 1) / kCandTile));
-        if (cand) atomicAdd(&scount[NUM_BUCKETS + 1], 1ull);
+        if (code == kCodeCand) atomicAdd(&scount[kSlotTiles + cand_class(d)], 1ull);
+        if (code == kCodeCandMt) atomicAdd(&scount[kSlotMt], 1ull);
+        if (cand) {
+            atomicAdd(&scount[kSlotCandV], 1ull);
+            // neighbour ids and gathered labels, offsets, own old and new label
+            atomicAdd(&sbytes, (unsigned long long)(8 * d + 2 * (int64_t)sizeof(E) + 8));
+        }
         scode[k] = code;
         sv[k] = (int32_t)v;
-        smt[k] = mt;
         sb[k] = b;
         sd[k] = d;
     }
     __syncthreads();
     // phase 2: one global atomic per non-empty list; LDS cursors start at its base
-    if (threadIdx.x < NUM_BUCKETS + 2) {
+    if (threadIdx.x < kSlots) {
         const int c = threadIdx.x;
-        const int gc = c < NUM_BUCKETS ? c : c == NUM_BUCKETS ? kCtrCandTiles : kCtrCandVerts;
+        const int gc = c < NUM_BUCKETS ? c
+                       : c < kSlotCandV ? kCtrCandTiles + (c - kSlotTiles)
+                       : c == kSlotCandV ? kCtrCandVerts : c == kSlotHubs ? kCtrPlainHubs : kCtrCandMt;
         const unsigned long long cnt = scount[c];
         scount[c] = cnt ? atomicAdd((unsigned long long *)(ctr + gc), cnt) : 0ull;
+        if (c >= kSlotTiles && c < kSlotCandV) scount[c] += (unsigned long long)cw_base[c - kSlotTiles];
     }
+    if (threadIdx.x == 64 && sbytes) atomicAdd((unsigned long long *)cand_bytes, sbytes);
     __syncthreads();
-    // phase 3: write the entries
+    // phase 3: write the entries (the tiles of multi-tile vertices are a fixed
+    // list made at set-up; the candidate kernel skips those not selected)
     for (int k = threadIdx.x; k < nloc; k += 256) {
         const uint8_t code = scode[k];
         const int32_t v = sv[k];
@@ -163,22 +189,11 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
             const unsigned long long q = atomicAdd(&scount[code], 1ull);
             plist[code][q] = v;
         } else if (code == kCodeCand) {
-            const unsigned long long q = atomicAdd(&scount[NUM_BUCKETS], 1ull);
+            const unsigned long long q = atomicAdd(&scount[kSlotTiles + cand_class(sd[k])], 1ull);
             cw_v[q] = v;
             cw_e0[q] = sb[k];
             cw_len[q] = (int32_t)sd[k];
             cw_pslot[q] = -1;
-        } else if (code == kCodeCandMt) {
-            const int64_t d = sd[k], b = sb[k];
-            const int nt = (int)((d + kCandTile - 1) / kCandTile);
-            const unsigned long long q = atomicAdd(&scount[NUM_BUCKETS], (unsigned long long)nt);
-            const int32_t ps = mt_ctile0[smt[k]];
-            for (int t = 0; t < nt; ++t) {
-                cw_v[q + t] = v;
-                cw_e0[q + t] = b + (int64_t)t * kCandTile;
-                cw_len[q + t] = (int32_t)min((int64_t)kCandTile, d - (int64_t)t * kCandTile);
-                cw_pslot[q + t] = ps + t;
-            }
         }
     }
 }
@@ -210,6 +225,8 @@ struct CandPolicy {
     int64_t *ctr;
     const int32_t *cache;   // LDS copy of labels [0, hot)
     int32_t hot;
+    const int32_t *tile_mt;   // fixed multi-tile list: tile -> multi-tile vertex (else null)
+    const int32_t *mt_sel;    // multi-tile vertex routed to candidates this iteration
     uint32_t changed;
 
     __device__ Item item(int64_t t) const {
@@ -222,6 +239,7 @@ struct CandPolicy {
             it.row0 = (uint32_t)(it.e0 - (int64_t)off[it.v]);
             it.old = prev[it.v];
             it.valid = true;
+            if (tile_mt && !mt_sel[tile_mt[t]]) it.len = 0;   // left to the class kernels
         }
         return it;
     }
@@ -231,6 +249,7 @@ struct CandPolicy {
     // (the driver's gather still issued a load, to prev[0] for them, which
     // all such lanes share, so it costs one line instead of one per lane)
     __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT]) {
+        if (it.len == 0) return;
         const int lane = lane_id();
         int ncand = 0;
         int32_t ck[kCandK];
@@ -366,26 +385,51 @@ __device__ __forceinline__ void cand_pipeline(Policy &pol, const int32_t *__rest
     }
 }
 
+// Candidate work lists, walked in turn by every wave of one launch (one
+// LDS cache fill): the fixed tiles of multi-tile vertices, then single-tile
+// vertices by length class.
+struct CandLists {
+    const int32_t *v[kCandClasses + 1], *len[kCandClasses + 1], *pslot[kCandClasses + 1];
+    const int64_t *e0[kCandClasses + 1];
+    const int32_t *tile_mt;   // fixed list: tile -> multi-tile vertex
+    int64_t nfixed;
+};
+
 template <typename E>
 __global__ void __launch_bounds__(1024)
 k_cand(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
-       int32_t *__restrict__ next, const int32_t *__restrict__ cw_v, const int64_t *__restrict__ cw_e0,
-       const int32_t *__restrict__ cw_len, const int32_t *__restrict__ cw_pslot, uint32_t *__restrict__ pkey,
+       int32_t *__restrict__ next, CandLists cl, const int32_t *__restrict__ mt_sel, uint32_t *__restrict__ pkey,
        uint32_t *__restrict__ pcnt, uint32_t *__restrict__ plast, int32_t *__restrict__ pn,
-       int32_t *__restrict__ dist_out, int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr,
-       int32_t hot, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
-    constexpr int EPT = kCandTile / kWave;
+       int32_t *__restrict__ dist_out, int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr, int32_t hot,
+       unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
     __shared__ int32_t cache[kCandCache];
     if (*stop) return;
+    const int64_t nmt = ctr[kCtrCandMt];
+    int64_t nt[kCandClasses];
+    int64_t any = nmt;
+#pragma unroll
+    for (int c = 0; c < kCandClasses; ++c) any += (nt[c] = ctr[kCtrCandTiles + c]);
+    if (any == 0) return;
     for (int i = threadIdx.x; i < hot; i += 1024) cache[i] = prev[i];
     __syncthreads();
-    const int64_t ntiles = ctr[kCtrCandTiles];
     const int64_t wave = (int64_t)blockIdx.x * 16 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int64_t nwaves = (int64_t)gridDim.x * 16;
-    CandPolicy<EPT, E> pol{off, prev,  next,  cw_v,     cw_len, cw_pslot, cw_e0, ntiles, pkey, pcnt,
-                           plast, pn,  dist_out, plist, ctr,    cache,    hot,   0u};
-    cand_pipeline<EPT>(pol, nbr, prev, wave, nwaves, hot);
-    const uint32_t s = wave_sum_u32(pol.changed);
+    uint32_t changed = 0;
+    auto run = [&](auto ept, int li, int64_t ntiles, const int32_t *tmt) {
+        constexpr int EPT = decltype(ept)::value;
+        if (ntiles == 0) return;
+        CandPolicy<EPT, E> pol{off,      prev,      next,  cl.v[li], cl.len[li], cl.pslot[li], cl.e0[li],
+                               ntiles,   pkey,      pcnt,  plast,    pn,         dist_out,     plist,
+                               ctr,      cache,     hot,   tmt,      mt_sel,     0u};
+        cand_pipeline<EPT>(pol, nbr, prev, wave, nwaves, hot);
+        changed += pol.changed;
+    };
+    if (nmt) run(std::integral_constant<int, 16>{}, kCandClasses, cl.nfixed, cl.tile_mt);
+    run(std::integral_constant<int, 16>{}, 3, nt[3], nullptr);
+    run(std::integral_constant<int, 8>{}, 2, nt[2], nullptr);
+    run(std::integral_constant<int, 4>{}, 1, nt[1], nullptr);
+    run(std::integral_constant<int, 2>{}, 0, nt[0], nullptr);
+    const uint32_t s = wave_sum_u32(changed);
     if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
@@ -409,7 +453,7 @@ k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int6
     __shared__ uint32_t nlist;
     __shared__ int32_t bad;
     __shared__ unsigned long long sred64[4];
-    if (*stop) return;
+    if (*stop || ctr[kCtrCandMt] == 0) return;
     uint32_t changed = 0;
     for (int s = threadIdx.x; s < CAP; s += 256) {
         skey[s] = kEmpty;
@@ -481,6 +525,7 @@ k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int6
             if (bad) {
                 if (h < nhubs) {
                     hub_sel[h] = 1;
+                    atomicAdd((unsigned long long *)(ctr + kCtrPlainHubs), 1ull);
                 } else {
                     const int c = bucket_of_deg((int64_t)off[v + 1] - (int64_t)off[v]);
                     const unsigned long long q = atomicAdd((unsigned long long *)(ctr + c), 1ull);
@@ -569,7 +614,8 @@ void learn_free(galp_graph *g) {
                     L->d_cw_v,    L->d_cw_len,     L->d_cw_pslot, L->d_cw_e0,   L->d_mt_vid,   L->d_mt_of,
                     L->d_mt_ctile0, L->d_mt_nctiles, L->d_mt_cand, L->d_pn,     L->d_pkey,     L->d_pcnt,
                     L->d_plast,   L->d_hub_sel,    L->d_hub_distinct, L->d_plist, L->d_plist_n, L->d_log_x,
-                    L->d_log_y,   L->d_log_n};
+                    L->d_log_y,   L->d_log_n, L->d_cand_bytes, L->d_cw_base,
+                    L->d_mt_tv,   L->d_mt_te0,     L->d_mt_tlen,  L->d_mt_tslot, L->d_mt_tmt};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int b = 0; b < NUM_BUCKETS; ++b)
@@ -597,6 +643,7 @@ int learn_setup(galp_graph *g) {
     GALP_LTRY(lalloc(g, &L->d_dist[0], n));
     GALP_LTRY(lalloc(g, &L->d_dist[1], n));
     GALP_LTRY(lalloc(g, &L->d_plist_n, kNumCtrs));
+    GALP_LTRY(lalloc(g, &L->d_cand_bytes, kMaxIters));
     GALP_LTRY(lalloc(g, &L->d_log_n, 1));
     for (int b = B_W128; b < B_HUB; ++b) GALP_LTRY(lalloc(g, &L->plist[b], g->bucket[b].nv));
     GALP_LTRY(lalloc(g, &L->d_plist, NUM_BUCKETS));
@@ -620,7 +667,17 @@ int learn_setup(galp_graph *g) {
             return fail(g->ctx, GALP_EINVAL, "internal: hub order");
     L->nmt = (int64_t)mt_vid.size();
     L->nctiles = nct;
-    L->cw_cap = (big_n - L->nmt) + nct;
+    {   // tile-class segments: single-tile vertices by length, multi-tile tiles in the last
+        int64_t segn[kCandClasses] = {};
+        for (size_t k = 0; k < g->big_deg.size(); ++k)
+            if (g->big_deg[k] <= kCandTile) segn[cand_class(g->big_deg[k])]++;
+        L->cw_base[0] = 0;
+        for (int c = 0; c < kCandClasses; ++c) L->cw_base[c + 1] = L->cw_base[c] + segn[c];
+    }
+    L->cw_cap = L->cw_base[kCandClasses];
+    GALP_LTRY(lalloc(g, &L->d_cw_base, kCandClasses + 1));
+    if (hipMemcpy(L->d_cw_base, L->cw_base, sizeof(L->cw_base), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(g->ctx, GALP_EHIP, "work-list segments upload failed");
     GALP_LTRY(lalloc(g, &L->d_cw_v, L->cw_cap));
     GALP_LTRY(lalloc(g, &L->d_cw_len, L->cw_cap));
     GALP_LTRY(lalloc(g, &L->d_cw_pslot, L->cw_cap));
@@ -630,6 +687,36 @@ int learn_setup(galp_graph *g) {
     GALP_LTRY(lalloc(g, &L->d_mt_ctile0, L->nmt));
     GALP_LTRY(lalloc(g, &L->d_mt_nctiles, L->nmt));
     GALP_LTRY(lalloc(g, &L->d_mt_cand, L->nmt));
+    {   // the fixed tile list of multi-tile vertices
+        std::vector<int32_t> tv, tlen, tslot, tmt;
+        std::vector<int64_t> te0;
+        for (size_t k = 0; k < g->big_vid.size(); ++k) {
+            const int64_t d = g->big_deg[k];
+            if (d <= kCandTile) continue;
+            const int32_t mt = mt_of[(size_t)g->big_vid[k]];
+            for (int64_t t = 0; t * kCandTile < d; ++t) {
+                tslot.push_back((int32_t)tv.size());
+                tv.push_back(g->big_vid[k]);
+                te0.push_back(g->big_off[k] + t * kCandTile);
+                tlen.push_back((int32_t)std::min<int64_t>(kCandTile, d - t * kCandTile));
+                tmt.push_back(mt);
+            }
+        }
+        GALP_LTRY(lalloc(g, &L->d_mt_tv, nct));
+        GALP_LTRY(lalloc(g, &L->d_mt_te0, nct));
+        GALP_LTRY(lalloc(g, &L->d_mt_tlen, nct));
+        GALP_LTRY(lalloc(g, &L->d_mt_tslot, nct));
+        GALP_LTRY(lalloc(g, &L->d_mt_tmt, nct));
+        bool ok = true;
+        if (nct) {
+            ok = hipMemcpy(L->d_mt_tv, tv.data(), 4 * tv.size(), hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(L->d_mt_te0, te0.data(), 8 * te0.size(), hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(L->d_mt_tlen, tlen.data(), 4 * tlen.size(), hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(L->d_mt_tslot, tslot.data(), 4 * tslot.size(), hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(L->d_mt_tmt, tmt.data(), 4 * tmt.size(), hipMemcpyHostToDevice) == hipSuccess;
+        }
+        if (!ok || (int64_t)tv.size() != nct) return fail(g->ctx, GALP_EHIP, "multi-tile list upload failed");
+    }
     GALP_LTRY(lalloc(g, &L->d_pn, nct));
     GALP_LTRY(lalloc(g, &L->d_pkey, nct * kCandK));
     GALP_LTRY(lalloc(g, &L->d_pcnt, nct * kCandK));
@@ -684,8 +771,9 @@ static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl) {
     if (!L->big_n) return;
     k_dispatch<H, E><<<(int)((L->big_n + kDispChunk - 1) / kDispChunk), 256, 0, g->stream>>>(
         (const E *)g->d_off, g->d_vlist, L->big_v0, L->big_n, L->d_params, L->d_fdeg, L->d_fmean, L->d_fmax,
-        L->d_dist[L->dcur], L->cand_thresh, g->nhubs, L->d_mt_of, L->d_mt_ctile0, L->d_mt_cand, L->d_hub_sel, L->d_cw_v,
-        L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, dpl, L->d_plist_n, g->d_stop);
+        L->d_dist[L->dcur], L->cand_thresh, g->nhubs, L->d_mt_of, L->d_mt_cand, L->d_hub_sel, L->d_cw_v,
+        L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, L->d_cw_base, dpl, L->d_plist_n, L->d_cand_bytes + g->iter % kMaxIters,
+        g->d_stop);
 }
 
 template <typename E>
@@ -695,7 +783,8 @@ static int learned_front(galp_graph *g) {
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
     PendingLaunch pl;
-    k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_plist_n);
+    int64_t *cb = L->d_cand_bytes + g->iter % kMaxIters;
+    k_reset_front<<<1, 64, 0, g->stream>>>(L->d_plist_n, cb);
     prof_begin(g, "dispatch", 0.0, &pl);
     switch (L->H) {
     case 8: launch_dispatch<8, E>(g, L, dpl); break;
@@ -703,12 +792,25 @@ static int learned_front(galp_graph *g) {
     default: launch_dispatch<32, E>(g, L, dpl); break;
     }
     prof_end(g, &pl);
-    double cand_bytes = 0.0;   // edges of the candidate path are not known on the host
-    prof_begin(g, "cand", cand_bytes, &pl);
+    prof_begin_dev(g, "cand", cb, &pl);   // the dispatch counts the candidate path's bytes
     const int32_t hot = g->reorder ? (int32_t)std::min<int64_t>(kCandCache, g->n) : 0;
+    CandLists cl{};
+    for (int c = 0; c < kCandClasses; ++c) {
+        const int64_t b = L->cw_base[c];
+        cl.v[c] = L->d_cw_v + b;
+        cl.len[c] = L->d_cw_len + b;
+        cl.pslot[c] = L->d_cw_pslot + b;
+        cl.e0[c] = L->d_cw_e0 + b;
+    }
+    cl.v[kCandClasses] = L->d_mt_tv;
+    cl.len[kCandClasses] = L->d_mt_tlen;
+    cl.pslot[kCandClasses] = L->d_mt_tslot;
+    cl.e0[kCandClasses] = L->d_mt_te0;
+    cl.tile_mt = L->d_mt_tmt;
+    cl.nfixed = L->nctiles;
     k_cand<E><<<g->ctx->prop.multiProcessorCount, 1024, 0, g->stream>>>(
-        (const E *)g->d_off, g->d_nbr, prev, next, L->d_cw_v, L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, L->d_pkey,
-        L->d_pcnt, L->d_plast, L->d_pn, L->distinct_out(g), dpl, L->d_plist_n, hot, g->d_counter, g->d_stop);
+        (const E *)g->d_off, g->d_nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn,
+        L->distinct_out(g), dpl, L->d_plist_n, hot, g->d_counter, g->d_stop);
     prof_end(g, &pl);
     if (L->nmt) {
         prof_begin(g, "cand_merge", 0.0, &pl);

@@ -409,14 +409,14 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
            const int64_t *__restrict__ tile_idx_off, int64_t ntiles,
            uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last,
            int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
-           const int32_t *__restrict__ hub_sel) {
+           const int32_t *__restrict__ hub_sel, const int64_t *__restrict__ nsel) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t slast[CAP];
     __shared__ uint32_t hist[kHubMaxParts + 1];
-    if (*stop) return;
+    if (*stop || (nsel && *nsel == 0)) return;   // no hub on this path
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
         scnt[s] = 0;
@@ -436,7 +436,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
             const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_cnt,
             const uint32_t *__restrict__ out_last, unsigned long long *__restrict__ hub_best,
             const int32_t *__restrict__ stop, int32_t *__restrict__ err,
-            const int32_t *__restrict__ hub_sel, uint32_t *__restrict__ hub_distinct) {
+            const int32_t *__restrict__ hub_sel, uint32_t *__restrict__ hub_distinct, const int64_t *__restrict__ nsel) {
     constexpr int CAP = 1 << kHubMergeCapBits;
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
@@ -445,7 +445,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     __shared__ uint32_t nlist;
     __shared__ int32_t slo[T], shi[T];
     __shared__ unsigned long long sred64[T / kWave];
-    if (*stop) return;
+    if (*stop || (nsel && *nsel == 0)) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
         scnt[s] = 0;
@@ -523,9 +523,9 @@ k_hub_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const in
             int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ hub_sel,
             uint32_t *__restrict__ hub_distinct, int32_t *__restrict__ distinct, const int32_t *__restrict__ prev,
             int32_t *__restrict__ next, unsigned long long *__restrict__ counter,
-            const int32_t *__restrict__ stop) {
+            const int32_t *__restrict__ stop, const int64_t *__restrict__ nsel) {
     __shared__ uint32_t sred[256 / kWave];
-    if (*stop) return;
+    if (*stop || (nsel && *nsel == 0)) return;
     uint32_t changed = 0;
     for (int64_t h = (int64_t)blockIdx.x * 256 + threadIdx.x; h < nhubs; h += (int64_t)gridDim.x * 256) {
         if (hub_sel && !hub_sel[h]) continue;
@@ -613,23 +613,24 @@ static int hash_iteration(galp_graph *g) {
         const BucketRange &r = g->bucket[B_HUB];
         const int32_t *sel = (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) ? g->learn->d_hub_sel : nullptr;
         uint32_t *hdist = g->learn ? g->learn->d_hub_distinct : nullptr;
+        const int64_t *nsel = sel ? g->learn->d_plist_n + kCtrPlainHubs : nullptr;
         PendingLaunch pl;
         prof_begin(g, "hub_tile", 8.0 * (double)r.edges, &pl);
         k_hub_tile<256, E><<<grid_for(g, g->nhub_tiles, 2), 256, 0, g->stream>>>(
             (const E *)g->d_off, g->d_nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
             g->d_tile_idx_off, g->nhub_tiles, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop, g->d_err,
-            sel);
+            sel, nsel);
         prof_end(g, &pl);
         prof_begin(g, "hub_merge", 0.0, &pl);
         k_hub_merge<256><<<grid_for(g, g->nhub_items, 3), 256, 0, g->stream>>>(
             g->d_item_hub, g->d_item_part, g->nhub_items, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
             g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, sel,
-            hdist);
+            hdist, nsel);
         prof_end(g, &pl);
         prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)r.nv + 16.0 * (double)r.nv, &pl);
         k_hub_final<E><<<grid_for(g, (g->nhubs + 255) / 256, 4), 256, 0, g->stream>>>(
             (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, sel, hdist,
-            g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter, g->d_stop);
+            g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter, g->d_stop, nsel);
         prof_end(g, &pl);
     }
     launch_lds<512, 8, 13, E>(g, B_B4K, prev, next, 2);

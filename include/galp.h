@@ -220,8 +220,9 @@ GALP_API int galp_graph_log_read(galp_graph *g, float *x, float *y, int64_t max,
  * neighbour degree, computed for vertices of degree > 64 (0 otherwise). */
 GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *mx);
 /* Counters of the last iteration's learned dispatch: [0..12) class list
- * sizes, [12] candidate tiles, [13] candidate overflows, [14] vertices
- * routed to candidates. */
+ * sizes, [12..16) candidate tiles per length class (<= 128, 256, 512, 1024
+ * edges), [16] candidate overflows, [17] vertices routed to candidates,
+ * [18] hubs left to the plain hub path. */
 GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max, int32_t *count);
 
 /* Message of the last failure on ctx; "" if none. */

@@ -155,9 +155,9 @@ def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, re
                                      f"oracle {ref[bad[0]]}")
             assert st["updated"] == cnt, (tag, it + 1, st["updated"], cnt)
             ds = g.dispatch_stats()
-            routed += int(ds[14])
-            overflowed += int(ds[13])
-            assert st["overflowed"] == ds[13]
+            routed += int(ds[_abi.DS_CAND_VERTS])
+            overflowed += int(ds[_abi.DS_OVERFLOW])
+            assert st["overflowed"] == ds[_abi.DS_OVERFLOW]
             prev = ref
             if st["stopped"]:
                 break
