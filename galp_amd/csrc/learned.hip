@@ -118,14 +118,12 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
            int32_t *__restrict__ cw_v, int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len,
            int32_t *__restrict__ cw_pslot, const int64_t *__restrict__ cw_base, int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr,
            int64_t *__restrict__ cand_bytes, const int32_t *__restrict__ stop) {
-    __shared__ float sp[mlp_num_params(H)];
     __shared__ uint8_t scode[kDispChunk];
     __shared__ int32_t sv[kDispChunk];
     __shared__ int64_t sb[kDispChunk], sd[kDispChunk];
     __shared__ unsigned long long scount[kSlots];
     __shared__ unsigned long long sbytes;
     if (*stop) return;
-    for (int i = threadIdx.x; i < mlp_num_params(H); i += 256) sp[i] = params[i];
     if (threadIdx.x < kSlots) scount[threadIdx.x] = 0;
     if (threadIdx.x == 0) sbytes = 0;
     __syncthreads();
@@ -139,7 +137,9 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
         const int64_t d = (int64_t)off[v + 1] - b;
         float x[4];
         mlp_inputs(fdeg[v], fmean[v], fmax[v], dprev[v], x);
-        const float y = mlp_forward<H>(sp, x[0], x[1], x[2], x[3]);
+        // weights through the scalar cache: wave-uniform addresses become
+        // scalar loads and SGPR operands of the FMAs (no LDS traffic)
+        const float y = mlp_forward<H>(params, x[0], x[1], x[2], x[3]);
         const bool cand = exp2f(y) - 1.f <= cand_thresh;
         const int cls = bucket_of_deg(d);
         const int mt = d > kCandTile ? mt_of[v] : -1;
