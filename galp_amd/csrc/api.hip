@@ -129,6 +129,7 @@ static int dev_alloc(galp_graph *g, T **p, int64_t count) {
 static void dev_free_all(galp_graph *g) {
     learn_free(g);
     sort_free(g);
+    ooc_free(g);
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
                     g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
@@ -322,12 +323,17 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     // ---- memory check (check_gpu_mem, main.cu:51-64) --------------------
     {
         const int64_t esz = g->off64 ? 8 : 4;
-        int64_t need = 4 * m + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
+        const bool ooc = g->opts.policy == GALP_POLICY_OOC_SYNC || g->opts.policy == GALP_POLICY_OOC_ASYNC;
+        // out-of-core: two batch buffers and two rebased offset arrays instead of the neighbour array
+        const int64_t resident = ooc ? 2 * 4 * std::min<int64_t>(m, (int64_t)1 << g->opts.buffer_pow) +
+                                           2 * esz * (n + 1)
+                                     : 4 * m;
+        int64_t need = resident + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
         if (g->opts.method == GALP_METHOD_HASH)
             need += 8 * g->nhub_items + (8 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
         else   // compact hub rows: keys and positions, twice; histograms and their scan
             need += 16 * g->bucket[B_HUB].edges + 12 * 256 * g->nhub_tiles;
-        if (g->reorder) need += 8 * n + 4 * m + 8 * (n + 1);  // perm arrays + transient original CSR
+        if (g->reorder) need += 8 * n + (ooc ? 0 : 4 * m + 8 * (n + 1));  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;
         GALP_HIP(ctx, hipMemGetInfo(&fr, &tot));
@@ -337,8 +343,9 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
                                               " bytes, available " + std::to_string(limit));
     }
 
+    const bool ooc = g->opts.policy == GALP_POLICY_OOC_SYNC || g->opts.policy == GALP_POLICY_OOC_ASYNC;
     GALP_HIP(ctx, hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-    GALP_TRY(dev_alloc(g, &g->d_nbr, m));
+    if (!ooc) GALP_TRY(dev_alloc(g, &g->d_nbr, m));
     GALP_TRY(dev_alloc(g, &g->d_lab[0], n));
     GALP_TRY(dev_alloc(g, &g->d_lab[1], n));
     GALP_TRY(dev_alloc(g, &g->d_counter, 1));
@@ -357,6 +364,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_out, n));
         GALP_HIP(ctx, hipMemcpy(g->d_perm, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
         GALP_HIP(ctx, hipMemcpy(g->d_iperm, iperm.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+    }
+    if (g->reorder && !ooc) {
         int64_t *d_off_old = nullptr;
         int32_t *d_nbr_old = nullptr;
         const int64_t before = g->device_bytes;
@@ -376,8 +385,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         g->device_bytes = before;
         if (rc != GALP_OK) return rc;
         if (e3 != hipSuccess) return fail(ctx, GALP_EHIP, std::string("permute: ") + hipGetErrorString(e3));
-    } else {
-        if (m) GALP_HIP(ctx, hipMemcpy(g->d_nbr, c->neighbors, 4 * (size_t)m, hipMemcpyHostToDevice));
+    } else if (!g->reorder) {
+        if (m && !ooc) GALP_HIP(ctx, hipMemcpy(g->d_nbr, c->neighbors, 4 * (size_t)m, hipMemcpyHostToDevice));
         if (!vlist.empty()) {
             GALP_TRY(dev_alloc(g, &g->d_vlist, (int64_t)vlist.size()));
             GALP_HIP(ctx, hipMemcpy(g->d_vlist, vlist.data(), 4 * vlist.size(), hipMemcpyHostToDevice));
@@ -413,6 +422,11 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         }
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
     }
+    if (ooc) {
+        std::vector<int64_t> hub_item0(hub_nparts.size() + 1, 0);
+        for (size_t h = 0; h < hub_nparts.size(); ++h) hub_item0[h + 1] = hub_item0[h] + hub_nparts[h];
+        GALP_TRY(ooc_setup(g, c, offi, perm, iperm, vlist, hub_vertices, hub_tile0, hub_item0));
+    }
     if (g->opts.method == GALP_METHOD_SORT) {
         std::vector<int64_t> hub_deg;
         for (int32_t x : hub_vertices) hub_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
@@ -424,6 +438,20 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     return GALP_OK;
 }
 
+IterView full_view(const galp_graph *g) {
+    IterView w;
+    w.off = g->d_off;
+    w.nbr = g->d_nbr;
+    for (int b = 0; b < NUM_BUCKETS; ++b) w.cls[b] = g->bucket[b];
+    w.h0 = 0;
+    w.h1 = g->nhubs;
+    w.t0 = 0;
+    w.t1 = g->nhub_tiles;
+    w.i0 = 0;
+    w.i1 = g->nhub_items;
+    return w;
+}
+
 static int launch_iteration(galp_graph *g) {
     switch (g->opts.method) {
     case GALP_METHOD_HASH:
@@ -432,11 +460,13 @@ static int launch_iteration(galp_graph *g) {
                 return fail(g->ctx, GALP_ESTATE, "learned dispatch needs a network: call galp_graph_set_mlp");
             GALP_TRY(launch_learned_front(g));
         }
-        GALP_TRY(launch_hash_iteration(g));
+        if (g->ooc) GALP_TRY(launch_ooc_iteration(g));
+        else GALP_TRY(launch_hash_iteration(g, full_view(g)));
         if (g->learn) GALP_TRY(learn_after_iteration(g));
         break;
     case GALP_METHOD_SORT:
-        GALP_TRY(launch_sort_iteration(g));
+        if (g->ooc) GALP_TRY(launch_ooc_iteration(g));
+        else GALP_TRY(launch_sort_iteration(g, full_view(g)));
         break;
     default:
         return fail(g->ctx, GALP_EUNSUPPORTED, "method " + std::to_string(g->opts.method) + " is not built");
@@ -524,8 +554,14 @@ GALP_API int galp_graph_upload(galp_ctx *ctx, const galp_csr_view *csr, const ga
     if (o.policy == 3) return fail(ctx, GALP_EUNSUPPORTED, "policy 3 (CPU-GPU hybrid) is not part of the one-GPU build");
     if (o.policy == 5 || (o.policy == 4 && o.method == GALP_METHOD_HASH))
         return fail(ctx, GALP_EUNSUPPORTED, "multi-GPU policies are not part of the one-GPU build");
-    if (o.policy != GALP_POLICY_INCORE && o.policy != GALP_POLICY_LI)
-        return fail(ctx, GALP_EUNSUPPORTED, "policy " + std::to_string(o.policy) + " is not built yet");
+    if (o.policy != GALP_POLICY_INCORE && o.policy != GALP_POLICY_LI && o.policy != GALP_POLICY_OOC_SYNC &&
+        o.policy != GALP_POLICY_OOC_ASYNC)
+        return fail(ctx, GALP_EUNSUPPORTED, "policy " + std::to_string(o.policy) + " is not built");
+    if ((o.policy == GALP_POLICY_OOC_SYNC || o.policy == GALP_POLICY_OOC_ASYNC) &&
+        (o.buffer_pow < 10 || o.buffer_pow > 34))
+        return fail(ctx, GALP_EINVAL, "buffer_pow (-b) must be in [10, 34] for the out-of-core policies");
+    if ((o.policy == GALP_POLICY_OOC_SYNC || o.policy == GALP_POLICY_OOC_ASYNC) && o.dispatch == GALP_DISPATCH_LEARNED)
+        return fail(ctx, GALP_EUNSUPPORTED, "learned dispatch is in-core only");
     if (o.variant < 0 || o.variant > 3) return fail(ctx, GALP_EINVAL, "variant (-l) must be 0..3");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, GALP_EHIP, "hipSetDevice failed");
     galp_graph *g = new (std::nothrow) galp_graph();

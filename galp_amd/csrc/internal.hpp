@@ -54,6 +54,18 @@ struct BucketRange {
 };
 
 
+// What one round of LP kernels works on: the whole graph in-core, or one
+// out-of-core batch (offsets rebased to the batch's neighbour buffer, each
+// class and the hub lists cut to the batch's vertices).
+struct IterView {
+    const void *off = nullptr;      // int32 or int64 offsets (rebased for a batch)
+    const int32_t *nbr = nullptr;   // neighbour ids (the batch buffer for a batch)
+    BucketRange cls[NUM_BUCKETS];   // v0/nv per class (v0: id, or index into vlist)
+    int64_t h0 = 0, h1 = 0;         // hubs [h0, h1)
+    int64_t t0 = 0, t1 = 0;         // their tiles
+    int64_t i0 = 0, i1 = 0;         // their merge items (hash method)
+};
+
 struct KernelAgg {
     std::string name;
     int64_t launches = 0;
@@ -150,10 +162,29 @@ struct SortState {
     uint32_t *d_hist = nullptr;
     int64_t *d_scan = nullptr, *d_bsum = nullptr;
     int64_t nscan = 0, nbsum = 0;
+    std::vector<int64_t> hoff_host;
 };
 
 
 
+}  // namespace galp
+
+namespace galp {
+// Out-of-core state (ooc.hip): the internal-numbering neighbour array in
+// pinned host memory, the batch plan, two device batch buffers with their
+// rebased offsets, and the copy stream and events of policy 2.
+struct OocState {
+    int32_t *h_nbr = nullptr;
+    std::vector<int64_t> bbs;            // vertex boundaries, #batches + 1
+    std::vector<int64_t> e0, e1;         // edge range of each batch
+    std::vector<IterView> views;         // each batch's share of the classes and hubs
+    int64_t bufcap = 0;                  // edges per device buffer
+    int32_t *d_buf[2] = {nullptr, nullptr};
+    void *d_offb[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+};
 }  // namespace galp
 
 struct galp_ctx {
@@ -211,6 +242,7 @@ struct galp_graph {
     galp::Profiler prof;
     galp::LearnState *learn = nullptr;
     galp::SortState *sort = nullptr;
+    galp::OocState *ooc = nullptr;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
     std::vector<int64_t> big_deg;
     std::vector<int64_t> big_off;      // row start of each big vertex
@@ -222,15 +254,25 @@ constexpr int kMaxIters = 4096;
 
 int fail(galp_ctx *ctx, int code, const std::string &msg);
 
-// Kernel launches for one LP iteration of the hash method (lp_hash.hip).
-int launch_hash_iteration(galp_graph *g);
+// The whole graph as one view (in-core).
+IterView full_view(const galp_graph *g);
+// Kernel launches for one LP round of the hash method over a view (lp_hash.hip).
+int launch_hash_iteration(galp_graph *g, const IterView &w);
 // Iteration bookkeeping: records the change count, sets the stop flag.
 int launch_iter_end(galp_graph *g);
 
 // Sort method (lp_sort.hip).
 int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg);
 void sort_free(galp_graph *g);
-int launch_sort_iteration(galp_graph *g);
+int launch_sort_iteration(galp_graph *g, const IterView &w);
+
+// Out-of-core policies (ooc.hip).
+int ooc_setup(galp_graph *g, const galp_csr_view *c, const std::vector<int64_t> &offi,
+              const std::vector<int32_t> &perm, const std::vector<int32_t> &iperm,
+              const std::vector<int32_t> &vlist, const std::vector<int32_t> &hub_vertices,
+              const std::vector<int32_t> &hub_tile0, const std::vector<int64_t> &hub_item0);
+void ooc_free(galp_graph *g);
+int launch_ooc_iteration(galp_graph *g);
 
 // Learned dispatch (learned.hip).
 int learn_setup(galp_graph *g);

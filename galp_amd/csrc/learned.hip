@@ -627,6 +627,7 @@ void learn_free(galp_graph *g) {
 // Allocates the learned-dispatch state and computes the static features.
 int learn_setup(galp_graph *g) {
     if (g->learn) return GALP_OK;
+    if (g->ooc) return fail(g->ctx, GALP_EUNSUPPORTED, "the learned dispatcher's features need the graph in-core");
     LearnState *L = new (std::nothrow) LearnState();
     if (!L) return fail(g->ctx, GALP_ENOMEM, "host allocation failed");
     g->learn = L;

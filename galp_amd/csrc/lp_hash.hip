@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(T)
 k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
            const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
            const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
-           const int64_t *__restrict__ tile_idx_off, int64_t ntiles,
+           const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles,
            uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last,
            int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
            const int32_t *__restrict__ hub_sel, const int64_t *__restrict__ nsel) {
@@ -425,7 +425,7 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
     HubTilePolicy<T, EPT, E> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles,  skey,
                                  scnt, slast,    hist,      out_key,  out_cnt,    out_last,     tile_idx, err,
                                  hub_sel};
-    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
 template <int T>
@@ -570,8 +570,8 @@ static double class_bytes(const BucketRange &r) {
 }
 
 template <int G, typename E>
-static void launch_group(galp_graph *g, int bid, const int32_t *prev, int32_t *next) {
-    const BucketRange &r = g->bucket[bid];
+static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next) {
+    const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
@@ -579,78 +579,83 @@ static void launch_group(galp_graph *g, int bid, const int32_t *prev, int32_t *n
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
     if (g->d_vlist)
         k_group<G, HashGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
     else
         k_group<G, HashGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
+            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
     prof_end(g, &pl);
 }
 
 template <int T, int EPT, int CAPBITS, typename E>
-static void launch_lds(galp_graph *g, int bid, const int32_t *prev, int32_t *next, int blocks_per_cu) {
-    const BucketRange &r = g->bucket[bid];
+static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
+                       int blocks_per_cu) {
+    const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
     int32_t *dist = g->learn ? g->learn->distinct_out(g) : nullptr;
     if (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED)   // this iteration's list for the class
         k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, next, g->learn->plist[bid], 0, r.nv, g->learn->d_plist_n + bid,
+            (const E *)w.off, w.nbr, prev, next, g->learn->plist[bid], 0, r.nv, g->learn->d_plist_n + bid,
             g->d_counter, g->d_stop, g->d_err, dist);
     else
         k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop,
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop,
             g->d_err, dist);
     prof_end(g, &pl);
 }
 
 template <typename E>
-static int hash_iteration(galp_graph *g) {
+static int hash_iteration(galp_graph *g, const IterView &w) {
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
     // largest work first
-    if (g->nhubs) {
-        const BucketRange &r = g->bucket[B_HUB];
+    if (w.h1 > w.h0) {
+        const int64_t nh = w.h1 - w.h0, nt = w.t1 - w.t0, ni = w.i1 - w.i0;
+        double hub_edges = 0.0;
+        if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
         const int32_t *sel = (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) ? g->learn->d_hub_sel : nullptr;
         uint32_t *hdist = g->learn ? g->learn->d_hub_distinct : nullptr;
         const int64_t *nsel = sel ? g->learn->d_plist_n + kCtrPlainHubs : nullptr;
         PendingLaunch pl;
-        prof_begin(g, "hub_tile", 8.0 * (double)r.edges, &pl);
-        k_hub_tile<256, E><<<grid_for(g, g->nhub_tiles, 2), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-            g->d_tile_idx_off, g->nhub_tiles, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop, g->d_err,
-            sel, nsel);
+        prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
+        // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
+        k_hub_tile<256, E><<<grid_for(g, nt, 2), 256, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
+            g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+            g->d_err, sel, nsel);
         prof_end(g, &pl);
         prof_begin(g, "hub_merge", 0.0, &pl);
-        k_hub_merge<256><<<grid_for(g, g->nhub_items, 3), 256, 0, g->stream>>>(
-            g->d_item_hub, g->d_item_part, g->nhub_items, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
+        k_hub_merge<256><<<grid_for(g, ni, 3), 256, 0, g->stream>>>(
+            g->d_item_hub + w.i0, g->d_item_part + w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
             g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, sel,
             hdist, nsel);
         prof_end(g, &pl);
-        prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)r.nv + 16.0 * (double)r.nv, &pl);
-        k_hub_final<E><<<grid_for(g, (g->nhubs + 255) / 256, 4), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, sel, hdist,
-            g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter, g->d_stop, nsel);
+        prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);
+        k_hub_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, sel ? sel + w.h0 : nullptr,
+            hdist ? hdist + w.h0 : nullptr, g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter,
+            g->d_stop, nsel);
         prof_end(g, &pl);
     }
-    launch_lds<512, 8, 13, E>(g, B_B4K, prev, next, 2);
-    launch_lds<256, 8, 12, E>(g, B_B2K, prev, next, 4);
-    launch_lds<128, 8, 11, E>(g, B_B1K, prev, next, 8);
-    launch_lds<64, 8, 10, E>(g, B_W512, prev, next, 16);
-    launch_lds<64, 4, 9, E>(g, B_W256, prev, next, 24);
-    launch_lds<64, 2, 8, E>(g, B_W128, prev, next, 32);
-    launch_group<64, E>(g, B_G64, prev, next);
-    launch_group<32, E>(g, B_G32, prev, next);
-    launch_group<16, E>(g, B_G16, prev, next);
-    launch_group<8, E>(g, B_G8, prev, next);
-    launch_group<4, E>(g, B_G4, prev, next);
+    launch_lds<512, 8, 13, E>(g, w, B_B4K, prev, next, 2);
+    launch_lds<256, 8, 12, E>(g, w, B_B2K, prev, next, 4);
+    launch_lds<128, 8, 11, E>(g, w, B_B1K, prev, next, 8);
+    launch_lds<64, 8, 10, E>(g, w, B_W512, prev, next, 16);
+    launch_lds<64, 4, 9, E>(g, w, B_W256, prev, next, 24);
+    launch_lds<64, 2, 8, E>(g, w, B_W128, prev, next, 32);
+    launch_group<64, E>(g, w, B_G64, prev, next);
+    launch_group<32, E>(g, w, B_G32, prev, next);
+    launch_group<16, E>(g, w, B_G16, prev, next);
+    launch_group<8, E>(g, w, B_G8, prev, next);
+    launch_group<4, E>(g, w, B_G4, prev, next);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
     return GALP_OK;
 }
 
-int launch_hash_iteration(galp_graph *g) {
-    return g->off64 ? hash_iteration<int64_t>(g) : hash_iteration<int32_t>(g);
+int launch_hash_iteration(galp_graph *g, const IterView &w) {
+    return g->off64 ? hash_iteration<int64_t>(g, w) : hash_iteration<int32_t>(g, w);
 }
 
 int launch_iter_end(galp_graph *g) {

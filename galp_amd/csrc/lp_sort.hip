@@ -241,11 +241,11 @@ __global__ void __launch_bounds__(kSortTileT)
 k_hs_gather_hist(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
                  const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ tile_hub,
                  const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
-                 const int64_t *__restrict__ hoff, int64_t ntiles, uint32_t *__restrict__ keys,
+                 const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles, uint32_t *__restrict__ keys,
                  uint32_t *__restrict__ hist, const int32_t *__restrict__ stop) {
     __shared__ uint32_t sh[256];
     if (*stop) return;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
         const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
         const int64_t rb = (int64_t)off[hub_vid[x.h]] + x.tl * kHubTile;
         sh[threadIdx.x] = 0;
@@ -274,12 +274,12 @@ k_hs_gather_hist(const E *__restrict__ off, const int32_t *__restrict__ nbr, con
 // Histogram of digit `shift` for passes >= 1.
 __global__ void __launch_bounds__(kSortTileT)
 k_hs_hist(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t ntiles,
+          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles,
           const uint32_t *__restrict__ keys, int shift, uint32_t *__restrict__ hist,
           const int32_t *__restrict__ stop) {
     __shared__ uint32_t sh[256];
     if (*stop) return;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
         const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
         sh[threadIdx.x] = 0;
         __syncthreads();
@@ -352,7 +352,7 @@ k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, int64_t *__restrict__ 
 }
 
 __global__ void __launch_bounds__(1024)
-k_scan_top(int64_t *__restrict__ bsum, int64_t nb, const int32_t *__restrict__ stop) {
+k_scan_top(int64_t *__restrict__ bsum, int64_t nb, int64_t base, const int32_t *__restrict__ stop) {
     __shared__ int64_t sw[1024 / kWave];
     if (*stop) return;
     const int64_t per = (nb + 1023) / 1024;
@@ -360,7 +360,7 @@ k_scan_top(int64_t *__restrict__ bsum, int64_t nb, const int32_t *__restrict__ s
     int64_t s = 0;
     for (int64_t i = b0; i < b0 + per && i < nb; ++i) s += bsum[i];
     int64_t total;
-    int64_t run = block_exclusive_sum<1024 / kWave>(s, sw, &total);
+    int64_t run = base + block_exclusive_sum<1024 / kWave>(s, sw, &total);
     for (int64_t i = b0; i < b0 + per && i < nb; ++i) {
         const int64_t v = bsum[i];
         bsum[i] = run;
@@ -408,7 +408,7 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool valid
 template <bool FIRST>
 __global__ void __launch_bounds__(kSortTileT)
 k_hs_scatter(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-             const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t ntiles,
+             const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles,
              const uint32_t *__restrict__ ksrc, const uint32_t *__restrict__ psrc, int shift,
              const int64_t *__restrict__ scan, uint32_t *__restrict__ kdst, uint32_t *__restrict__ pdst,
              const int32_t *__restrict__ stop) {
@@ -420,7 +420,7 @@ k_hs_scatter(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ h
     const int lane = lane_id();
     const int w = (int)(threadIdx.x / kWave);
     const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
         const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
         for (int i = threadIdx.x; i < NW * 256; i += kSortTileT) (&whist[0][0])[i] = 0;
         uint32_t kv[CH], pv[CH];
@@ -478,14 +478,14 @@ k_hs_scatter(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ h
 // the tile's best (count << 32 | last) per tile.
 __global__ void __launch_bounds__(kSortTileT)
 k_hs_runs(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t ntiles,
+          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles,
           const uint32_t *__restrict__ keys, const uint32_t *__restrict__ pos,
           unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop) {
     constexpr int NW = kSortTileT / kWave;
     __shared__ int64_t sw[NW];
     __shared__ unsigned long long sred64[NW];
     if (*stop) return;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
         const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
         const int64_t hb = hoff[x.h], he = hoff[x.h + 1];
         const int j0 = (int)threadIdx.x * kSortEPT;
@@ -618,6 +618,7 @@ int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg) {
     std::vector<int64_t> hoff(hub_deg.size() + 1, 0);
     for (size_t h = 0; h < hub_deg.size(); ++h) hoff[h + 1] = hoff[h] + hub_deg[h];
     S->eh = hoff.back();
+    S->hoff_host = hoff;
     if (!g->nhubs) return GALP_OK;
     S->nscan = g->nhub_tiles * 256;
     S->nbsum = (S->nscan + kScanChunk - 1) / kScanChunk;
@@ -639,8 +640,8 @@ static double class_bytes(const BucketRange &r) {
 }
 
 template <int G, typename E>
-static void sort_group(galp_graph *g, int bid, const int32_t *prev, int32_t *next) {
-    const BucketRange &r = g->bucket[bid];
+static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next) {
+    const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
@@ -648,96 +649,105 @@ static void sort_group(galp_graph *g, int bid, const int32_t *prev, int32_t *nex
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
     if (g->d_vlist)
         k_group<G, SortGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
     else
         k_group<G, SortGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
-            (const E *)g->d_off, g->d_nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
+            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
     prof_end(g, &pl);
 }
 
 template <int T, int EPT, typename E>
-static void sort_lds(galp_graph *g, int bid, const int32_t *prev, int32_t *next, int blocks_per_cu) {
-    const BucketRange &r = g->bucket[bid];
+static void sort_lds(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
+                     int blocks_per_cu) {
+    const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
     k_sort_lds<T, EPT, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-        (const E *)g->d_off, g->d_nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
+        (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
     prof_end(g, &pl);
 }
 
+// Hubs [h0, h1) with tiles [t0, t1): the histogram, scan and scatter of every
+// pass cover just those tiles; the scan starts at the hubs' first compact
+// slot, so the output lands at the same places as in a whole-graph pass.
 template <typename E>
-static int sort_hubs(galp_graph *g, const int32_t *prev, int32_t *next) {
+static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next) {
     SortState *S = g->sort;
-    const int64_t nt = g->nhub_tiles;
+    const int64_t t0 = w.t0, t1 = w.t1, nt = t1 - t0, nh = w.h1 - w.h0;
     const int gt = grid_for(g, nt, 8);
-    const int nbs = (int)S->nbsum;
+    const int64_t nscan = nt * 256;
+    const int nbs = (int)((nscan + kScanChunk - 1) / kScanChunk);
+    uint32_t *hist = S->d_hist + t0 * 256;
+    int64_t *scan = S->d_scan + t0 * 256;
+    double hub_edges = 0.0;
+    if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
     PendingLaunch pl;
-    prof_begin(g, "hub_sort", 8.0 * (double)g->bucket[B_HUB].edges, &pl);
-    k_hs_gather_hist<E><<<gt, kSortTileT, 0, g->stream>>>((const E *)g->d_off, g->d_nbr, prev, g->d_hub_vid,
+    prof_begin(g, "hub_sort", 8.0 * hub_edges, &pl);
+    k_hs_gather_hist<E><<<gt, kSortTileT, 0, g->stream>>>((const E *)w.off, w.nbr, prev, g->d_hub_vid,
                                                           g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff,
-                                                          nt, S->d_kA, S->d_hist, g->d_stop);
+                                                          t0, t1, S->d_kA, S->d_hist, g->d_stop);
     uint32_t *ks = S->d_kA, *ps = S->d_pA, *kd = S->d_kB, *pd = S->d_pB;
     for (int p = 0; p < S->passes; ++p) {
         if (p > 0)
-            k_hs_hist<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, nt,
-                                                        ks, 8 * p, S->d_hist, g->d_stop);
-        k_scan_reduce<<<nbs, kScanT, 0, g->stream>>>(S->d_hist, S->nscan, S->d_bsum, g->d_stop);
-        k_scan_top<<<1, 1024, 0, g->stream>>>(S->d_bsum, S->nbsum, g->d_stop);
-        k_scan_down<<<nbs, kScanT, 0, g->stream>>>(S->d_hist, S->nscan, S->d_bsum, S->d_scan, g->d_stop);
+            k_hs_hist<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, t0,
+                                                        t1, ks, 8 * p, S->d_hist, g->d_stop);
+        k_scan_reduce<<<nbs, kScanT, 0, g->stream>>>(hist, nscan, S->d_bsum, g->d_stop);
+        k_scan_top<<<1, 1024, 0, g->stream>>>(S->d_bsum, nbs, S->hoff_host[(size_t)w.h0], g->d_stop);
+        k_scan_down<<<nbs, kScanT, 0, g->stream>>>(hist, nscan, S->d_bsum, scan, g->d_stop);
         if (p == 0)
             k_hs_scatter<true><<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles,
-                                                                 S->d_hoff, nt, ks, nullptr, 0, S->d_scan, kd, pd,
+                                                                 S->d_hoff, t0, t1, ks, nullptr, 0, S->d_scan, kd, pd,
                                                                  g->d_stop);
         else
             k_hs_scatter<false><<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles,
-                                                                  S->d_hoff, nt, ks, ps, 8 * p, S->d_scan, kd, pd,
-                                                                  g->d_stop);
+                                                                  S->d_hoff, t0, t1, ks, ps, 8 * p, S->d_scan, kd,
+                                                                  pd, g->d_stop);
         std::swap(ks, kd);
         std::swap(ps, pd);
     }
-    k_hs_runs<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, nt, ks, ps,
-                                                 g->d_hub_best, g->d_stop);
-    k_hs_final<E><<<grid_for(g, (g->nhubs + 255) / 256, 4), 256, 0, g->stream>>>(
-        (const E *)g->d_off, g->d_nbr, g->d_hub_vid, g->nhubs, g->d_hub_best, prev, next, g->d_counter, g->d_stop);
+    k_hs_runs<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, t0, t1,
+                                                 ks, ps, g->d_hub_best, g->d_stop);
+    k_hs_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
+        (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop);
     prof_end(g, &pl);
     return GALP_OK;
 }
 
 template <typename E>
-static int sort_iteration(galp_graph *g) {
+static int sort_iteration(galp_graph *g, const IterView &w) {
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
-    if (g->nhubs) {
-        const int rc = sort_hubs<E>(g, prev, next);
+    if (w.h1 > w.h0) {
+        const int rc = sort_hubs<E>(g, w, prev, next);
         if (rc != GALP_OK) return rc;
     }
     if (g->opts.policy == GALP_POLICY_LI) {
         // "load-imbalanced" ablation: every class <= 4096 through the
         // 512-thread block-per-vertex kernel, as the reference's li path
         // runs one block per vertex whatever its degree.
-        for (int b = B_B4K; b >= B_G4; --b) sort_lds<512, 8, E>(g, b, prev, next, 2);
+        for (int b = B_B4K; b >= B_G4; --b) sort_lds<512, 8, E>(g, w, b, prev, next, 2);
     } else {
-        sort_lds<512, 8, E>(g, B_B4K, prev, next, 2);
-        sort_lds<256, 8, E>(g, B_B2K, prev, next, 4);
-        sort_lds<128, 8, E>(g, B_B1K, prev, next, 8);
-        sort_lds<64, 8, E>(g, B_W512, prev, next, 16);
-        sort_lds<64, 4, E>(g, B_W256, prev, next, 24);
-        sort_lds<64, 2, E>(g, B_W128, prev, next, 32);
-        sort_group<64, E>(g, B_G64, prev, next);
-        sort_group<32, E>(g, B_G32, prev, next);
-        sort_group<16, E>(g, B_G16, prev, next);
-        sort_group<8, E>(g, B_G8, prev, next);
-        sort_group<4, E>(g, B_G4, prev, next);
+        sort_lds<512, 8, E>(g, w, B_B4K, prev, next, 2);
+        sort_lds<256, 8, E>(g, w, B_B2K, prev, next, 4);
+        sort_lds<128, 8, E>(g, w, B_B1K, prev, next, 8);
+        sort_lds<64, 8, E>(g, w, B_W512, prev, next, 16);
+        sort_lds<64, 4, E>(g, w, B_W256, prev, next, 24);
+        sort_lds<64, 2, E>(g, w, B_W128, prev, next, 32);
+        sort_group<64, E>(g, w, B_G64, prev, next);
+        sort_group<32, E>(g, w, B_G32, prev, next);
+        sort_group<16, E>(g, w, B_G16, prev, next);
+        sort_group<8, E>(g, w, B_G8, prev, next);
+        sort_group<4, E>(g, w, B_G4, prev, next);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("sort kernel launch: ") + hipGetErrorString(e));
     return GALP_OK;
 }
 
-int launch_sort_iteration(galp_graph *g) {
+int launch_sort_iteration(galp_graph *g, const IterView &w) {
     if (!g->sort) return fail(g->ctx, GALP_ESTATE, "internal: sort state missing");
-    return g->off64 ? sort_iteration<int64_t>(g) : sort_iteration<int32_t>(g);
+    return g->off64 ? sort_iteration<int64_t>(g, w) : sort_iteration<int32_t>(g, w);
 }
 
 }  // namespace galp

@@ -163,6 +163,30 @@ def test_device_loop_equals_steps(ctx, method):
     assert np.array_equal(labels, final)
 
 
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("policy", [_abi.POLICY_OOC_SYNC, _abi.POLICY_OOC_ASYNC])
+@pytest.mark.parametrize("reorder", REORDER)
+def test_out_of_core(ctx, method, policy, reorder):
+    """Policies 1 and 2 stream the neighbour array in batches of < 2^b edges;
+    2^14 puts the 40000-edge hub in a batch of its own and cuts the rest into
+    many batches.  Labels equal the oracle's (and so the in-core ones)."""
+    off, nbr = hub_graph(3)
+    check_per_iteration(ctx, off, nbr, 5, f"ooc-p{policy}", method=method, policy=policy, reorder=reorder,
+                        buffer_pow=14)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_out_of_core_batch_sizes(ctx, method):
+    off, nbr = _abi.gen_rmat(15, 16, seed=7)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    for b in (10, 13, 17, 24):
+        for policy in (_abi.POLICY_OOC_SYNC, _abi.POLICY_OOC_ASYNC):
+            with _abi.Graph(ctx, off, nbr, method=method, policy=policy, buffer_pow=b) as g:
+                labels, res = g.run(10)
+            assert res["iterations"] == len(per), (b, policy)
+            assert np.array_equal(labels, final), (b, policy)
+
+
 def test_sort_li_policy(ctx):
     """Policy 4 (li): method 0 with every class up to 4096 in the
     block-per-vertex kernel gives the same labels."""
