@@ -99,11 +99,23 @@ static void prof_collect(galp_graph *g) {
     g->prof.pending.clear();
 }
 
-static int bucket_of(int64_t d) {
+// Degree class of a vertex; rows longer than hub_min go to the hub path.
+static int bucket_of(int64_t d, int64_t hub_min) {
     if (d <= 0) return -1;
+    if (d > hub_min) return B_HUB;
     for (int b = 0; b < NUM_BUCKETS; ++b)
         if (d <= kBucketHi[b]) return b;
     return NUM_BUCKETS - 1;
+}
+
+// -l of method 1 (main.cu:98-100): which rows take the tiled hub path.
+//   2 (small LDS tables + spill, lfht/kernel.cuh:141-190): rows > 512;
+//   3 (load-balanced tiles, lfht/kernel.cuh:193-304): rows > 1024;
+//   0, 1 and method 0: rows > 4096.
+static int64_t hub_min_of(const galp_opts &o) {
+    if (o.method == GALP_METHOD_HASH && o.variant == 2) return 512;
+    if (o.method == GALP_METHOD_HASH && o.variant == 3) return 1024;
+    return kBucketHi[B_B4K];
 }
 
 template <class T>
@@ -238,7 +250,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         for (int64_t k = 0; k < n; ++k) off_int[(size_t)k + 1] = off_int[(size_t)k] + deg[(size_t)iperm[(size_t)k]];
         for (int b = 0; b < NUM_BUCKETS; ++b) g->bucket[b] = BucketRange{};
         for (int64_t k = 0; k < n; ++k) {
-            int b = bucket_of(deg[(size_t)iperm[(size_t)k]]);
+            int b = bucket_of(deg[(size_t)iperm[(size_t)k]], hub_min_of(g->opts));
             if (b < 0) continue;
             BucketRange &r = g->bucket[b];
             if (r.nv == 0) r.v0 = k;
@@ -249,7 +261,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     } else {
         std::vector<int64_t> cnt(NUM_BUCKETS, 0);
         for (int64_t v = 0; v < n; ++v) {
-            int b = bucket_of(deg[(size_t)v]);
+            int b = bucket_of(deg[(size_t)v], hub_min_of(g->opts));
             if (b >= 0) cnt[(size_t)b]++;
         }
         int64_t acc = 0;
@@ -262,7 +274,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         std::vector<int64_t> cur(NUM_BUCKETS);
         for (int b = 0; b < NUM_BUCKETS; ++b) cur[(size_t)b] = g->bucket[b].v0;
         for (int64_t v = 0; v < n; ++v) {
-            int b = bucket_of(deg[(size_t)v]);
+            int b = bucket_of(deg[(size_t)v], hub_min_of(g->opts));
             if (b < 0) continue;
             vlist[(size_t)cur[(size_t)b]++] = (int32_t)v;
             g->bucket[b].nv++;
@@ -562,6 +574,8 @@ GALP_API int galp_graph_upload(galp_ctx *ctx, const galp_csr_view *csr, const ga
         return fail(ctx, GALP_EINVAL, "buffer_pow (-b) must be in [10, 34] for the out-of-core policies");
     if ((o.policy == GALP_POLICY_OOC_SYNC || o.policy == GALP_POLICY_OOC_ASYNC) && o.dispatch == GALP_DISPATCH_LEARNED)
         return fail(ctx, GALP_EUNSUPPORTED, "learned dispatch is in-core only");
+    if (o.dispatch == GALP_DISPATCH_LEARNED && (o.method != GALP_METHOD_HASH || o.variant != 1))
+        return fail(ctx, GALP_EUNSUPPORTED, "learned dispatch routes method 1, -l 1");
     if (o.variant < 0 || o.variant > 3) return fail(ctx, GALP_EINVAL, "variant (-l) must be 0..3");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, GALP_EHIP, "hipSetDevice failed");
     galp_graph *g = new (std::nothrow) galp_graph();

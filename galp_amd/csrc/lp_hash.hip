@@ -638,6 +638,15 @@ static int hash_iteration(galp_graph *g, const IterView &w) {
             g->d_stop, nsel);
         prof_end(g, &pl);
     }
+    if (g->opts.variant == 0) {
+        // -l 0: no degree classes below the hubs -- every row through the
+        // block-per-vertex table kernel, as the reference's naive variant
+        // gives every vertex the same treatment
+        for (int b = B_B4K; b >= B_G4; --b) launch_lds<512, 8, 13, E>(g, w, b, prev, next, 2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
+        return GALP_OK;
+    }
     launch_lds<512, 8, 13, E>(g, w, B_B4K, prev, next, 2);
     launch_lds<256, 8, 12, E>(g, w, B_B2K, prev, next, 4);
     launch_lds<128, 8, 11, E>(g, w, B_B1K, prev, next, 8);

@@ -187,6 +187,16 @@ def test_out_of_core_batch_sizes(ctx, method):
             assert np.array_equal(labels, final), (b, policy)
 
 
+@pytest.mark.parametrize("variant", [0, 2, 3])
+def test_hash_variants(ctx, variant):
+    """-l 0 (no degree classes), 2 (rows > 512 on the tiled path) and 3
+    (rows > 1024 tiled) change only which kernels run."""
+    off, nbr = hub_graph(4)
+    check_per_iteration(ctx, off, nbr, 4, f"l{variant}", method=_abi.METHOD_HASH, variant=variant)
+    off, nbr = _abi.gen_rmat(14, 16, seed=9)
+    check_per_iteration(ctx, off, nbr, 4, f"l{variant}-rmat14", method=_abi.METHOD_HASH, variant=variant, reorder=0)
+
+
 def test_sort_li_policy(ctx):
     """Policy 4 (li): method 0 with every class up to 4096 in the
     block-per-vertex kernel gives the same labels."""
