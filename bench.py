@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--variants", default="hash_learned,hash_plain,sort")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
+    ap.add_argument("--save-mlp", help="write the trained dispatcher network (float32 .npy)")
+    ap.add_argument("--load-mlp", help="use this network instead of training one (float32 .npy)")
+    ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
+                    help="only copy MIB MiB device-to-device (a reference kernel for the HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -109,6 +113,16 @@ def main():
 
     import numpy as np
     from galp_amd import _abi
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 4 * a.numel(), "launches": 3}), flush=True)
+        return
 
     t = time.perf_counter()
     off, nbr = _abi.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=args.scale)
@@ -121,9 +135,16 @@ def main():
     mlp = None
     train_info = None
     if "hash_learned" in names:
-        mlp, nsamples, loss = train_network(ctx, off, nbr, args.iters)
-        train_info = {"samples": nsamples, "loss": round(loss, 5), "hidden": MLP_HIDDEN,
-                      "cand_thresh": CAND_THRESH}
+        if args.load_mlp:
+            mlp = np.load(args.load_mlp).astype(np.float32)
+            train_info = {"loaded": os.path.basename(args.load_mlp), "hidden": MLP_HIDDEN,
+                          "cand_thresh": CAND_THRESH}
+        else:
+            mlp, nsamples, loss = train_network(ctx, off, nbr, args.iters)
+            train_info = {"samples": nsamples, "loss": round(loss, 5), "hidden": MLP_HIDDEN,
+                          "cand_thresh": CAND_THRESH}
+        if args.save_mlp:
+            np.save(args.save_mlp, mlp)
     table = {"hash_learned": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_LEARNED),
              "hash_plain": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_PLAIN),
              "sort": dict(method=_abi.METHOD_SORT, dispatch=_abi.DISPATCH_PLAIN)}
