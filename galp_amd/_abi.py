@@ -42,7 +42,7 @@ class CsrView(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("policy", ctypes.c_int32), ("variant", ctypes.c_int32),
                 ("buffer_pow", ctypes.c_int32), ("dispatch", ctypes.c_int32), ("reorder", ctypes.c_int32),
-                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32), ("mem_budget", ctypes.c_int64)]
+                ("profile", ctypes.c_int32), ("graph_capture", ctypes.c_int32), ("mem_budget", ctypes.c_int64)]
 
 
 class StepStats(ctypes.Structure):
@@ -132,6 +132,8 @@ def lib():
         "galp_graph_log_read": (ctypes.c_int, [vp, vp, vp, i64, ctypes.POINTER(i64)]),
         "galp_graph_features": (ctypes.c_int, [vp, vp, vp, vp]),
         "galp_graph_dispatch_stats": (ctypes.c_int, [vp, vp, i32, ctypes.POINTER(i32)]),
+        "galp_graph_set_batch": (ctypes.c_int, [vp, i64, vp]),
+        "galp_batch_iterations": (ctypes.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -238,11 +240,12 @@ class Context:
 
 
 def make_opts(method=METHOD_HASH, policy=POLICY_INCORE, variant=1, buffer_pow=24, dispatch=DISPATCH_PLAIN,
-              reorder=1, profile=0, mem_budget=0):
+              reorder=1, profile=0, mem_budget=0, graph_capture=0):
     o = Opts()
     lib().galp_opts_default(ctypes.byref(o))
     o.method, o.policy, o.variant, o.buffer_pow = method, policy, variant, buffer_pow
     o.dispatch, o.reorder, o.profile, o.mem_budget = dispatch, reorder, profile, mem_budget
+    o.graph_capture = graph_capture
     return o
 
 
@@ -315,6 +318,17 @@ class Graph:
     def profile_reset(self):
         self.ctx.check(lib().galp_profile_reset(self.h))
 
+
+    # ---- batched runs ------------------------------------------------------
+    def set_batch(self, vbase):
+        vb = np.ascontiguousarray(vbase, np.int64)
+        self.ngraphs = len(vb) - 1
+        self.ctx.check(lib().galp_graph_set_batch(self.h, self.ngraphs, vb.ctypes.data))
+
+    def batch_iterations(self):
+        out = np.zeros(self.ngraphs, np.int32)
+        self.ctx.check(lib().galp_batch_iterations(self.h, out.ctypes.data))
+        return out
 
     # ---- learned dispatch --------------------------------------------------
     def set_mlp(self, hidden, params, cand_thresh):

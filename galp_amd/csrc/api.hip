@@ -142,6 +142,9 @@ static void dev_free_all(galp_graph *g) {
     learn_free(g);
     sort_free(g);
     ooc_free(g);
+    batch_free(g);
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+    g->gexec = nullptr;
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
                     g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
@@ -483,7 +486,48 @@ static int launch_iteration(galp_graph *g) {
     default:
         return fail(g->ctx, GALP_EUNSUPPORTED, "method " + std::to_string(g->opts.method) + " is not built");
     }
-    return launch_iter_end(g);
+    return g->batch ? launch_batch_end(g) : launch_iter_end(g);
+}
+
+// The iteration loop of galp_lp_run as one hipGraph: captured on the
+// graph's stream the first time a run of `niter` iterations is asked for,
+// replayed afterwards.  The captured launches hold the same arguments the
+// eager loop passes (ping-pong buffers, iteration slots); every run starts
+// from galp_lp_init, so the replay starts from the state the capture saw.
+// The loop records no memset node: every reset is a kernel.
+static int run_captured(galp_graph *g, int niter) {
+    if (!g->gexec || g->gexec_niter != niter) {
+        if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+        g->gexec = nullptr;
+        const int cur0 = g->cur;
+        const int dcur0 = g->learn ? g->learn->dcur : 0;
+        GALP_HIP(g->ctx, hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        int rc = GALP_OK;
+        for (int i = 0; i < niter && rc == GALP_OK; ++i) {
+            g->iter = i;
+            rc = launch_iteration(g);
+            g->cur ^= 1;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+        g->cur = cur0;
+        if (g->learn) g->learn->dcur = dcur0;
+        if (rc != GALP_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g->gexec = nullptr;
+            return fail(g->ctx, GALP_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        g->gexec_niter = niter;
+    }
+    GALP_HIP(g->ctx, hipGraphLaunch(g->gexec, g->stream));
+    if (g->learn && (niter & 1)) g->learn->dcur ^= 1;   // as the eager loop leaves it
+    return GALP_OK;
 }
 
 static int check_err(galp_graph *g) {
@@ -626,6 +670,7 @@ GALP_API int galp_lp_init(galp_graph *g) {
     if (!g) return GALP_EINVAL;
     GALP_TRY(dev_init_labels(g));
     GALP_TRY(learn_reset_iteration(g));
+    GALP_TRY(batch_reset(g));
     GALP_HIP(g->ctx, hipStreamSynchronize(g->stream));
     g->cur = 0;
     g->iter = 0;
@@ -640,6 +685,7 @@ GALP_API int galp_lp_set_labels(galp_graph *g, const int32_t *host_labels) {
             return fail(g->ctx, GALP_EINVAL, "labels must lie in [0, n)");
     GALP_TRY(dev_init_labels(g));   // resets the control words
     GALP_TRY(learn_reset_iteration(g));
+    GALP_TRY(batch_reset(g));
     if (g->n) {
         int32_t *dst = g->d_lab[0];
         if (g->reorder) {
@@ -744,10 +790,14 @@ GALP_API int galp_lp_run(galp_graph *g, int32_t niter, int32_t verbose, int32_t 
         int rc = dev_clear_stop(g);
         (void)hipEventRecord(e0, g->stream);
         const int cur0 = g->cur;
-        for (int i = 0; i < niter && rc == GALP_OK; ++i) {
-            g->iter = i;
-            rc = launch_iteration(g);
-            g->cur ^= 1;
+        if (g->opts.graph_capture && !g->ooc && !g->prof.on && niter > 0) {
+            rc = run_captured(g, niter);
+        } else {
+            for (int i = 0; i < niter && rc == GALP_OK; ++i) {
+                g->iter = i;
+                rc = launch_iteration(g);
+                g->cur ^= 1;
+            }
         }
         (void)hipEventRecord(e1, g->stream);
         if (rc == GALP_OK) {

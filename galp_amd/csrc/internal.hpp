@@ -187,6 +187,18 @@ struct OocState {
 };
 }  // namespace galp
 
+namespace galp {
+// Batched runs (batch.hip): graph id per vertex (internal numbering), per
+// graph vertex count, change counter, stop flag and iterations done.
+struct BatchState {
+    int64_t ngraphs = 0;
+    int32_t *d_gid = nullptr;
+    int64_t *d_gn = nullptr, *d_vbase = nullptr;
+    unsigned long long *d_gcount = nullptr;
+    int32_t *d_gstop = nullptr, *d_giters = nullptr;
+};
+}  // namespace galp
+
 struct galp_ctx {
     int device = 0;
     hipDeviceProp_t prop{};
@@ -243,6 +255,9 @@ struct galp_graph {
     galp::LearnState *learn = nullptr;
     galp::SortState *sort = nullptr;
     galp::OocState *ooc = nullptr;
+    galp::BatchState *batch = nullptr;
+    hipGraphExec_t gexec = nullptr;    // captured iteration loop (opts.graph_capture)
+    int gexec_niter = -1;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
     std::vector<int64_t> big_deg;
     std::vector<int64_t> big_off;      // row start of each big vertex
@@ -273,6 +288,18 @@ int ooc_setup(galp_graph *g, const galp_csr_view *c, const std::vector<int64_t> 
               const std::vector<int32_t> &hub_tile0, const std::vector<int64_t> &hub_item0);
 void ooc_free(galp_graph *g);
 int launch_ooc_iteration(galp_graph *g);
+
+// Drops the captured iteration loop after a change to what it launches.
+inline void invalidate_captured(galp_graph *g) {
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+    g->gexec = nullptr;
+    g->gexec_niter = -1;
+}
+
+// Batched runs (batch.hip).
+void batch_free(galp_graph *g);
+int batch_reset(galp_graph *g);
+int launch_batch_end(galp_graph *g);
 
 // Learned dispatch (learned.hip).
 int learn_setup(galp_graph *g);

@@ -853,6 +853,7 @@ GALP_API int galp_graph_set_mlp(galp_graph *g, int32_t hidden, const float *para
     if (!(cand_thresh >= 0.f)) return fail(g->ctx, GALP_EINVAL, "cand_thresh must be >= 0");
     int rc = learn_setup(g);
     if (rc != GALP_OK) return rc;
+    invalidate_captured(g);
     LearnState *L = g->learn;
     if (!L->d_params) {
         if (hipMalloc((void **)&L->d_params, 4 * (size_t)mlp_num_params(32)) != hipSuccess)
@@ -869,6 +870,7 @@ GALP_API int galp_graph_log_enable(galp_graph *g, int64_t capacity) {
     if (!g || capacity < 0) return GALP_EINVAL;
     int rc = learn_setup(g);
     if (rc != GALP_OK) return rc;
+    invalidate_captured(g);
     LearnState *L = g->learn;
     if (L->d_log_x) (void)hipFree(L->d_log_x);
     if (L->d_log_y) (void)hipFree(L->d_log_y);

@@ -72,7 +72,9 @@ typedef struct galp_opts {
     int32_t reorder;     /* 1 (default): degree-descending internal renumbering;
                             labels are mapped back on every read-out */
     int32_t profile;     /* 1: time every kernel launch with HIP events */
-    int32_t reserved;
+    int32_t graph_capture; /* 1: galp_lp_run (verbose = 0, in-core, profile = 0)
+                              captures its iteration loop in a hipGraph once per
+                              niter and replays it */
     int64_t mem_budget;  /* device bytes the graph may allocate; 0 = free memory.
                             Exceeding it fails with GALP_ENOMEM (check_gpu_mem,
                             main.cu:51-64) */
@@ -224,6 +226,20 @@ GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *
  * edges), [16] candidate overflows, [17] vertices routed to candidates,
  * [18] hubs left to the plain hub path. */
 GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max, int32_t *count);
+
+/* ---- batched runs --------------------------------------------------------
+ * Declares the uploaded (in-core) CSR to be the disjoint union of ngraphs
+ * graphs, graph k owning vertices [vbase[k], vbase[k+1]) of the original
+ * numbering (vbase[0] = 0, vbase[ngraphs] = n; an edge joining two graphs is
+ * rejected).  Each graph then follows its own stop rule (stop after an
+ * iteration with updated_k <= n_k * 1e-5), so graph k's labels equal an
+ * independent run of it shifted by vbase[k], while one launch per degree
+ * class serves all graphs.  galp_step_stats.updated and the per-iteration
+ * counts sum over the graphs still running.  No reference counterpart (the
+ * reference runs one graph per process). */
+GALP_API int galp_graph_set_batch(galp_graph *g, int64_t ngraphs, const int64_t *vbase);
+/* Iterations each graph ran in the last run (ngraphs entries). */
+GALP_API int galp_batch_iterations(galp_graph *g, int32_t *iters_out);
 
 /* Message of the last failure on ctx; "" if none. */
 GALP_API const char *galp_last_error(galp_ctx *ctx);
