@@ -61,6 +61,77 @@ def train_network(ctx, off, nbr, niter):
     return p, len(x), loss
 
 
+def make_batch(count, seed=4096):
+    """Planted-partition graphs of uneven size (a scaled-down batched config:
+    n ~ U[1000, 20000], community size ~ U[32, 1024], intra degree
+    ~ U[8, 48], inter = round(intra / 4)), concatenated as one disjoint-union
+    CSR.  Generated on host threads (the generator releases the GIL)."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    from galp_amd import _abi
+    rng = np.random.default_rng(seed)
+    specs = [(int(rng.integers(1000, 20001)), int(rng.integers(32, 1025)), int(rng.integers(8, 49)))
+             for _ in range(count)]
+
+    def gen(k):
+        n, comm, intra = specs[k]
+        return _abi.gen_planted_partition(n, comm, intra, max(1, round(intra / 4)), seed * 100000 + k)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        parts = list(ex.map(gen, range(count)))
+    vbase = np.zeros(count + 1, np.int64)
+    vbase[1:] = np.cumsum([len(o) - 1 for o, _ in parts])
+    ebase = np.zeros(count + 1, np.int64)
+    ebase[1:] = np.cumsum([len(nb) for _, nb in parts])
+    off = np.concatenate([np.zeros(1, np.int64)] + [o[1:] + ebase[k] for k, (o, _) in enumerate(parts)])
+    nbr = np.concatenate([nb + np.int32(vbase[k]) for k, (_, nb) in enumerate(parts)]).astype(np.int32)
+    return parts, vbase, off, nbr
+
+
+def time_batched(ctx, count, steps, niter):
+    """One batched run of `count` graphs (one launch per degree class, the
+    10-iteration loop captured in a hipGraph) against running the same graphs
+    one by one; labels of the batched run checked against the one-by-one
+    ones."""
+    import numpy as np
+    from galp_amd import _abi
+    t = time.perf_counter()
+    parts, vbase, off, nbr = make_batch(count)
+    m = len(nbr)
+    log(f"bench: batched: {count} graphs, n={len(off) - 1} m={m} generated in {time.perf_counter() - t:.1f}s")
+    with _abi.Graph(ctx, off, nbr, reorder=0, graph_capture=1) as g:
+        g.set_batch(vbase)
+        g.run(niter, want_labels=False)
+        iters = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _, res = g.run(niter, want_labels=False)
+            iters += int(g.batch_iterations().sum())
+        wall = time.perf_counter() - t0
+        labels, _ = g.run(niter)
+        per_graph_iters = g.batch_iterations()
+    # the same graphs one by one (each its own upload, launches and loop)
+    edges_iters, t1, ok = 0, 0.0, True
+    for k, (o, nb) in enumerate(parts):
+        with _abi.Graph(ctx, o, nb, reorder=0) as gk:
+            gk.run(niter, want_labels=False)
+            t0 = time.perf_counter()
+            lab, res = gk.run(niter)
+            t1 += time.perf_counter() - t0
+            edges_iters += len(nb) * res["iterations"]
+            ok = ok and np.array_equal(lab, labels[vbase[k]:vbase[k + 1]] - vbase[k]) and \
+                res["iterations"] == per_graph_iters[k]
+    edge_iters_batched = sum(len(nb) * int(per_graph_iters[k]) for k, (_, nb) in enumerate(parts)) * steps
+    return {"graphs": count, "n": int(len(off) - 1), "m": int(m),
+            "gteps": round(edge_iters_batched / wall / 1e9, 3),
+            "one_by_one_gteps": round(edges_iters / t1 / 1e9, 3),
+            "ms_per_run": round(wall / steps * 1e3, 3),
+            "labels_equal_one_by_one": bool(ok),
+            "workload": f"{count} planted-partition graphs, n~U[1000,20000], community~U[32,1024], "
+                        f"intra~U[8,48], inter=intra/4, seeds 4096*100000+k; hash method, plain dispatch, "
+                        f"hipGraph-captured {niter}-iteration loop, per-graph stop rule"}
+
+
 def time_variant(ctx, off, nbr, steps, warmup, niter, mlp=None, **opts):
     """Times `steps` runs without instrumentation (the value), then the same
     runs again with a HIP event pair around every kernel launch (the
@@ -100,6 +171,8 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--variants", default="hash_learned,hash_plain,sort")
+    ap.add_argument("--batch-graphs", type=int, default=512,
+                    help="graphs of the batched measurement (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--save-mlp", help="write the trained dispatcher network (float32 .npy)")
     ap.add_argument("--load-mlp", help="use this network instead of training one (float32 .npy)")
@@ -203,6 +276,9 @@ def main():
         "device": {"name": dev["name"], "arch": dev["arch"], "cus": dev["compute_units"]},
         "preprocess_ms": round(main_v["info"]["preprocess_ms"], 1),
     }
+    if args.batch_graphs:
+        out["batched"] = time_batched(ctx, args.batch_graphs, args.steps, args.iters)
+        log(f"bench: batched: {out['batched']}")
     if "hash_learned" in vg and "hash_plain" in vg:
         out["learned_vs_plain"] = round(vg["hash_learned"] / vg["hash_plain"], 3)
         out["learned_network"] = train_info
