@@ -133,6 +133,8 @@ def lib():
         "galp_graph_features": (ctypes.c_int, [vp, vp, vp, vp]),
         "galp_graph_dispatch_stats": (ctypes.c_int, [vp, vp, i32, ctypes.POINTER(i32)]),
         "galp_graph_set_batch": (ctypes.c_int, [vp, i64, vp]),
+        "galp_graph_autotune": (ctypes.c_int, [vp, i32, ctypes.c_char_p, vp]),
+        "galp_graph_set_tuning": (ctypes.c_int, [vp, vp]),
         "galp_batch_iterations": (ctypes.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -318,6 +320,18 @@ class Graph:
     def profile_reset(self):
         self.ctx.check(lib().galp_profile_reset(self.h))
 
+
+    # ---- autotuner ---------------------------------------------------------
+    def autotune(self, reps=3, cache_path=None):
+        out = np.zeros(16, np.int32)
+        self.ctx.check(lib().galp_graph_autotune(self.h, reps, os.fsencode(cache_path) if cache_path else None,
+                                                 out.ctypes.data))
+        return out[:12]
+
+    def set_tuning(self, options):
+        o = np.ascontiguousarray(options, np.int32)
+        assert len(o) == 12
+        self.ctx.check(lib().galp_graph_set_tuning(self.h, o.ctypes.data))
 
     # ---- batched runs ------------------------------------------------------
     def set_batch(self, vbase):

@@ -256,6 +256,7 @@ struct galp_graph {
     galp::SortState *sort = nullptr;
     galp::OocState *ooc = nullptr;
     galp::BatchState *batch = nullptr;
+    int8_t tune[galp::NUM_BUCKETS] = {};   // kernel configuration per degree class (autotuner)
     hipGraphExec_t gexec = nullptr;    // captured iteration loop (opts.graph_capture)
     int gexec_niter = -1;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
@@ -273,6 +274,8 @@ int fail(galp_ctx *ctx, int code, const std::string &msg);
 IterView full_view(const galp_graph *g);
 // Kernel launches for one LP round of the hash method over a view (lp_hash.hip).
 int launch_hash_iteration(galp_graph *g, const IterView &w);
+// One class kernel of the hash method over the whole graph (autotuner).
+int launch_hash_class(galp_graph *g, int b);
 // Iteration bookkeeping: records the change count, sets the stop flag.
 int launch_iter_end(galp_graph *g);
 

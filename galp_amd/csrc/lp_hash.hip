@@ -570,7 +570,8 @@ static double class_bytes(const BucketRange &r) {
 }
 
 template <int G, typename E>
-static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next) {
+static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
+                         int blocks_per_cu = 8) {
     const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
@@ -578,10 +579,10 @@ static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_
     const int64_t per_block = 256 / G;
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
     if (g->d_vlist)
-        k_group<G, HashGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+        k_group<G, HashGroupResolve, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
     else
-        k_group<G, HashGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+        k_group<G, HashGroupResolve, false, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
     prof_end(g, &pl);
 }
@@ -603,6 +604,65 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
             (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop,
             g->d_err, dist);
     prof_end(g, &pl);
+}
+
+// The kernel configuration of each degree class: option 0 is the default,
+// 1 and 2 the alternatives the autotuner times (workgroup size x slots per
+// lane, and blocks per CU for the grid; for lane-group classes the grid
+// only).  Every option computes the same labels.
+template <typename E>
+static void launch_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
+    const int c = g->tune[b];
+    switch (b) {
+    case B_B4K:
+        if (c == 1) launch_lds<1024, 4, 13, E>(g, w, b, prev, next, 1);
+        else if (c == 2) launch_lds<256, 16, 13, E>(g, w, b, prev, next, 4);
+        else launch_lds<512, 8, 13, E>(g, w, b, prev, next, 2);
+        break;
+    case B_B2K:
+        if (c == 1) launch_lds<512, 4, 12, E>(g, w, b, prev, next, 3);
+        else if (c == 2) launch_lds<128, 16, 12, E>(g, w, b, prev, next, 8);
+        else launch_lds<256, 8, 12, E>(g, w, b, prev, next, 4);
+        break;
+    case B_B1K:
+        if (c == 1) launch_lds<256, 4, 11, E>(g, w, b, prev, next, 6);
+        else if (c == 2) launch_lds<64, 16, 11, E>(g, w, b, prev, next, 16);
+        else launch_lds<128, 8, 11, E>(g, w, b, prev, next, 8);
+        break;
+    case B_W512:
+        if (c == 1) launch_lds<128, 4, 10, E>(g, w, b, prev, next, 12);
+        else if (c == 2) launch_lds<256, 2, 10, E>(g, w, b, prev, next, 8);
+        else launch_lds<64, 8, 10, E>(g, w, b, prev, next, 16);
+        break;
+    case B_W256:
+        if (c == 1) launch_lds<128, 2, 9, E>(g, w, b, prev, next, 16);
+        else if (c == 2) launch_lds<64, 4, 9, E>(g, w, b, prev, next, 12);
+        else launch_lds<64, 4, 9, E>(g, w, b, prev, next, 24);
+        break;
+    case B_W128:
+        if (c == 1) launch_lds<128, 1, 8, E>(g, w, b, prev, next, 16);
+        else if (c == 2) launch_lds<64, 2, 8, E>(g, w, b, prev, next, 16);
+        else launch_lds<64, 2, 8, E>(g, w, b, prev, next, 32);
+        break;
+    default: {   // lane groups: blocks per CU of the persistent grid
+        const int bpc = c == 1 ? 16 : c == 2 ? 4 : 8;
+        if (b == B_G64) launch_group<64, E>(g, w, b, prev, next, bpc);
+        else if (b == B_G32) launch_group<32, E>(g, w, b, prev, next, bpc);
+        else if (b == B_G16) launch_group<16, E>(g, w, b, prev, next, bpc);
+        else if (b == B_G8) launch_group<8, E>(g, w, b, prev, next, bpc);
+        else launch_group<4, E>(g, w, b, prev, next, bpc);
+    }
+    }
+}
+
+int launch_hash_class(galp_graph *g, int b) {
+    const IterView w = full_view(g);
+    const int32_t *prev = g->d_lab[g->cur];
+    int32_t *next = g->d_lab[g->cur ^ 1];
+    if (g->off64) launch_class<int64_t>(g, w, b, prev, next);
+    else launch_class<int32_t>(g, w, b, prev, next);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("class launch: ") + hipGetErrorString(e));
 }
 
 template <typename E>
@@ -647,17 +707,7 @@ static int hash_iteration(galp_graph *g, const IterView &w) {
         if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
         return GALP_OK;
     }
-    launch_lds<512, 8, 13, E>(g, w, B_B4K, prev, next, 2);
-    launch_lds<256, 8, 12, E>(g, w, B_B2K, prev, next, 4);
-    launch_lds<128, 8, 11, E>(g, w, B_B1K, prev, next, 8);
-    launch_lds<64, 8, 10, E>(g, w, B_W512, prev, next, 16);
-    launch_lds<64, 4, 9, E>(g, w, B_W256, prev, next, 24);
-    launch_lds<64, 2, 8, E>(g, w, B_W128, prev, next, 32);
-    launch_group<64, E>(g, w, B_G64, prev, next);
-    launch_group<32, E>(g, w, B_G32, prev, next);
-    launch_group<16, E>(g, w, B_G16, prev, next);
-    launch_group<8, E>(g, w, B_G8, prev, next);
-    launch_group<4, E>(g, w, B_G4, prev, next);
+    for (int b = B_B4K; b >= B_G4; --b) launch_class<E>(g, w, b, prev, next);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
     return GALP_OK;

@@ -1,0 +1,69 @@
+"""The autotuner: every kernel configuration of every degree class gives the
+oracle's labels; autotuning picks valid options, keeps the graph
+initialised, and round-trips its choices through the cache file."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from galp_amd import _abi
+from test_gpu_parity import check_per_iteration, hub_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("option", [0, 1, 2])
+@pytest.mark.parametrize("reorder", [1, 0])
+def test_every_option_same_labels(ctx, option, reorder):
+    off, nbr = hub_graph(5)
+    # extra rows of 600..4000 edges so B1K..B4K have vertices
+    rng = np.random.default_rng(option)
+    n = len(off) - 1
+    rows = [nbr[off[v]:off[v + 1]].tolist() for v in range(n)]
+    for v in range(10, 60):
+        rows[v] += rng.integers(10, n, size=int(rng.integers(600, 4000))).tolist()
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(r) for r in rows])
+    nbr = np.array([u for r in rows for u in r], np.int32)
+    print(f"[tune] option {option} reorder {reorder}", flush=True)
+    with _abi.Graph(ctx, off, nbr, reorder=reorder) as g:
+        g.set_tuning([option] * 12)
+        g.init()
+        prev = np.arange(n, dtype=np.int32)
+        for it in range(4):
+            ref, cnt = oracle.lp_iterate(off, nbr, prev)
+            st = g.step()
+            assert np.array_equal(g.labels(), ref), it
+            assert st["updated"] == cnt
+            prev = ref
+
+
+def test_autotune_and_cache(ctx, tmp_path):
+    off, nbr = _abi.gen_rmat(16, 16, seed=21)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    cache = str(tmp_path / "tune.txt")
+    with _abi.Graph(ctx, off, nbr) as g:
+        chosen = g.autotune(reps=2, cache_path=cache)
+        assert all(0 <= c <= 2 for c in chosen)
+        labels, res = g.run(10)
+        assert np.array_equal(labels, final)
+    lines = open(cache).read().split("\n")
+    assert len([x for x in lines if x.strip()]) >= 5
+    with _abi.Graph(ctx, off, nbr) as g2:
+        chosen2 = g2.autotune(reps=2, cache_path=cache)   # every class hits the cache
+        assert np.array_equal(chosen, chosen2)
+        labels, _ = g2.run(10)
+        assert np.array_equal(labels, final)
+    assert open(cache).read().split("\n") == lines      # nothing new appended
+
+
+def test_autotune_learned_graph(ctx):
+    off, nbr = _abi.gen_rmat(15, 16, seed=22)
+    final, _, _ = oracle.lp_run(off, nbr, niter=10)
+    with _abi.Graph(ctx, off, nbr, dispatch=_abi.DISPATCH_LEARNED) as g:
+        g.set_mlp(8, np.zeros(_abi.mlp_num_params(8), np.float32), 4.0)
+        g.autotune(reps=1)
+        labels, _ = g.run(10)
+        assert np.array_equal(labels, final)
