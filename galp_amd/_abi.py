@@ -42,7 +42,8 @@ class CsrView(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("policy", ctypes.c_int32), ("variant", ctypes.c_int32),
                 ("buffer_pow", ctypes.c_int32), ("dispatch", ctypes.c_int32), ("reorder", ctypes.c_int32),
-                ("profile", ctypes.c_int32), ("graph_capture", ctypes.c_int32), ("mem_budget", ctypes.c_int64)]
+                ("profile", ctypes.c_int32), ("graph_capture", ctypes.c_int32), ("mem_budget", ctypes.c_int64),
+                ("offsets64", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class StepStats(ctypes.Structure):
@@ -242,12 +243,13 @@ class Context:
 
 
 def make_opts(method=METHOD_HASH, policy=POLICY_INCORE, variant=1, buffer_pow=24, dispatch=DISPATCH_PLAIN,
-              reorder=1, profile=0, mem_budget=0, graph_capture=0):
+              reorder=1, profile=0, mem_budget=0, graph_capture=0, offsets64=0):
     o = Opts()
     lib().galp_opts_default(ctypes.byref(o))
     o.method, o.policy, o.variant, o.buffer_pow = method, policy, variant, buffer_pow
     o.dispatch, o.reorder, o.profile, o.mem_budget = dispatch, reorder, profile, mem_budget
     o.graph_capture = graph_capture
+    o.offsets64 = offsets64
     return o
 
 

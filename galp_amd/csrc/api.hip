@@ -216,7 +216,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     const int64_t n = c->n, m = c->m;
     g->n = n;
     g->m = m;
-    g->off64 = m > (int64_t)INT32_MAX;
+    g->off64 = m > (int64_t)INT32_MAX || g->opts.offsets64 != 0;
     g->reorder = g->opts.reorder != 0;
 
     std::vector<int64_t> deg((size_t)n);

@@ -78,6 +78,9 @@ typedef struct galp_opts {
     int64_t mem_budget;  /* device bytes the graph may allocate; 0 = free memory.
                             Exceeding it fails with GALP_ENOMEM (check_gpu_mem,
                             main.cu:51-64) */
+    int32_t offsets64;   /* 1: int64 device offsets even when m < 2^31 (they
+                            are int64 whenever m >= 2^31) */
+    int32_t reserved;
 } galp_opts;
 
 /* Per-iteration result.  Replaces the count returned by iterate()/get_count()

@@ -197,6 +197,28 @@ def test_hash_variants(ctx, variant):
     check_per_iteration(ctx, off, nbr, 4, f"l{variant}-rmat14", method=_abi.METHOD_HASH, variant=variant, reorder=0)
 
 
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("policy", [_abi.POLICY_INCORE, _abi.POLICY_OOC_ASYNC])
+def test_int64_offsets(ctx, method, policy):
+    """The int64-offset kernels (used when m >= 2^31) on graphs small enough
+    for the oracle, by forcing int64 offsets."""
+    off, nbr = hub_graph(6)
+    check_per_iteration(ctx, off, nbr, 4, "off64-hubs", method=method, policy=policy, offsets64=1, buffer_pow=14)
+    off, nbr = _abi.gen_rmat(15, 16, seed=13)
+    check_per_iteration(ctx, off, nbr, 4, "off64-rmat15", method=method, policy=policy, offsets64=1, reorder=0)
+
+
+def test_int64_offsets_learned_batched_captured(ctx):
+    off, nbr = _abi.gen_rmat(15, 16, seed=14)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    p = np.zeros(_abi.mlp_num_params(16), np.float32)
+    with _abi.Graph(ctx, off, nbr, dispatch=_abi.DISPATCH_LEARNED, offsets64=1, graph_capture=1) as g:
+        g.set_mlp(16, p, 6.0)
+        for rep in range(2):
+            labels, res = g.run(10)
+            assert np.array_equal(labels, final), rep
+
+
 def test_sort_li_policy(ctx):
     """Policy 4 (li): method 0 with every class up to 4096 in the
     block-per-vertex kernel gives the same labels."""
