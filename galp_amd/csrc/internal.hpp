@@ -104,7 +104,7 @@ constexpr int kNumCtrs = kCtrOverflow + 4;
 // Candidate path of the learned dispatcher: a wave walks a tile of up to
 // kCandTile edges keeping up to kCandK labels in registers.
 constexpr int kCandTile = 1024;
-constexpr int kCandK = 8;
+constexpr int kCandK = 16;     // candidates per tile (lanes of the candidate table, <= 64)
 constexpr int kCandCache = 32768;   // hottest labels held in LDS per workgroup
 
 // State of the learned dispatcher (allocated when dispatch is learned or

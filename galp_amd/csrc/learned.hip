@@ -4,8 +4,9 @@
 // of degree > 64 ("big" vertices) and routes it:
 //   - predicted distinct labels <= cand_thresh: the candidate path -- one
 //     wave per tile of <= kCandTile edges keeps up to kCandK (label, count,
-//     last position) candidates in registers and matches each 64-edge chunk
-//     against them with ballots: no LDS table, so the workgroup's LDS holds
+//     last position) candidates in the lanes of three registers and peels
+//     each 64-edge chunk label by label with ballots: no LDS table, so the
+//     workgroup's LDS holds
 //     the kCandCache hottest labels (the lowest ids after the degree-
 //     descending renumbering) and gathers that hit them skip L2;
 //   - otherwise: the plain degree-class kernels of lp_hash.hip, through
@@ -248,93 +249,80 @@ struct CandPolicy {
     // lab holds prev[u] for cold neighbours; hot ones are re-read from LDS
     // (the driver's gather still issued a load, to prev[0] for them, which
     // all such lanes share, so it costs one line instead of one per lane)
+    // Candidates live in lanes: lane j < ncand holds candidate j's label,
+    // count and last position.  Each 64-edge chunk is peeled: one round per
+    // distinct label present in the chunk (ballot of the equal lanes gives
+    // its count and last position), looked up among the candidates with one
+    // more ballot.  A converged row costs one round per chunk.
     __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT]) {
         if (it.len == 0) return;
         const int lane = lane_id();
-        int ncand = 0;
-        int32_t ck[kCandK];
-        uint32_t cc[kCandK], cl[kCandK];
-#pragma unroll
-        for (int j = 0; j < kCandK; ++j) {
-            ck[j] = -1;
-            cc[j] = 0;
-            cl[j] = 0;
-        }
-        bool overflow = false;
+        int32_t ckv = -1;          // candidate label of this lane (-1: none)
+        uint32_t ccv = 0, clv = 0;
+        int ncand = 0;             // uniform
+        bool overflow = false;     // uniform
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const int pos = k * kWave + lane;
             if (k * kWave >= it.len || overflow) break;   // uniform
-            const bool ok = pos < it.len;
+            const int pos = k * kWave + lane;
             const int32_t u = ids[k];
-            const int32_t L = ok ? (u < hot ? cache[u] : lab[k]) : -1;
+            const int32_t L = pos < it.len ? (u < hot ? cache[u] : lab[k]) : -1;
             const uint32_t rowpos = it.row0 + (uint32_t)(k * kWave);
-            bool matched = false;
-#pragma unroll
-            for (int j = 0; j < kCandK; ++j) {
-                if (j < ncand) {
-                    const bool eq = L == ck[j];
-                    const unsigned long long m = __ballot(eq);
-                    if (m) {
-                        cc[j] += (uint32_t)__popcll(m);
-                        cl[j] = rowpos + (uint32_t)(63 - __clzll((long long)m));
+            unsigned long long ball = __builtin_amdgcn_ballot_w64(L >= 0);
+            while (ball) {
+                const int32_t Lk = __builtin_amdgcn_readlane(L, __ffsll((long long)ball) - 1);
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(L == Lk);
+                ball &= ~m;
+                const uint32_t c = (uint32_t)__popcll(m);
+                const uint32_t lastp = rowpos + (uint32_t)(63 - __clzll((long long)m));
+                const unsigned long long hit = __builtin_amdgcn_ballot_w64(ckv == Lk);
+                if (hit) {
+                    if (lane == __ffsll((long long)hit) - 1) {
+                        ccv += c;
+                        clv = lastp;
                     }
-                    matched = matched || eq;
-                }
-            }
-            unsigned long long rest = __ballot(ok && !matched);
-            while (rest) {
-                if (ncand == kCandK) {
-                    overflow = true;
-                    break;
-                }
-                const int32_t c = __builtin_amdgcn_readlane(L, __ffsll((long long)rest) - 1);
-                const unsigned long long m = __ballot(ok && L == c);
-#pragma unroll
-                for (int j = 0; j < kCandK; ++j)
-                    if (j == ncand) {
-                        ck[j] = c;
-                        cc[j] = (uint32_t)__popcll(m);
-                        cl[j] = rowpos + (uint32_t)(63 - __clzll((long long)m));
+                } else {
+                    if (ncand == kCandK) {
+                        overflow = true;
+                        break;
                     }
-                ++ncand;
-                rest &= ~m;
+                    if (lane == ncand) {
+                        ckv = Lk;
+                        ccv = c;
+                        clv = lastp;
+                    }
+                    ++ncand;
+                }
             }
         }
-        if (lane != 0) return;
-        if (it.pslot >= 0) {   // hub tile: partial candidates
-            pn[it.pslot] = overflow ? -1 : ncand;
-            if (!overflow)
-#pragma unroll
-                for (int j = 0; j < kCandK; ++j)
-                    if (j < ncand) {
-                        pkey[(int64_t)it.pslot * kCandK + j] = (uint32_t)ck[j];
-                        pcnt[(int64_t)it.pslot * kCandK + j] = cc[j];
-                        plast[(int64_t)it.pslot * kCandK + j] = cl[j];
-                    }
+        if (it.pslot >= 0) {   // tile of a multi-tile vertex: partial candidates
+            if (lane == 0) pn[it.pslot] = overflow ? -1 : ncand;
+            if (!overflow && lane < ncand) {
+                const int64_t e = (int64_t)it.pslot * kCandK + lane;
+                pkey[e] = (uint32_t)ckv;
+                pcnt[e] = ccv;
+                plast[e] = clv;
+            }
             return;
         }
         if (overflow) {   // the plain class kernel of this degree takes the vertex
-            const int c = bucket_of_deg(it.len);
-            const unsigned long long q = atomicAdd((unsigned long long *)(ctr + c), 1ull);
-            plist[c][q] = it.v;
-            atomicAdd((unsigned long long *)(ctr + kCtrOverflow), 1ull);
+            if (lane == 0) {
+                const int c = bucket_of_deg(it.len);
+                const unsigned long long q = atomicAdd((unsigned long long *)(ctr + c), 1ull);
+                plist[c][q] = it.v;
+                atomicAdd((unsigned long long *)(ctr + kCtrOverflow), 1ull);
+            }
             return;
         }
-        unsigned long long best = 0;
-        int32_t w = 0;
-#pragma unroll
-        for (int j = 0; j < kCandK; ++j)
-            if (j < ncand) {
-                const unsigned long long pk = ((unsigned long long)cc[j] << 32) | cl[j];
-                if (pk > best) {
-                    best = pk;
-                    w = ck[j];
-                }
-            }
-        next[it.v] = w;
-        changed += (w != it.old);
-        if (dist_out) dist_out[it.v] = ncand;
+        const unsigned long long val = lane < ncand ? (((unsigned long long)ccv << 32) | clv) : 0ull;
+        const unsigned long long top = wave_max(val);
+        const unsigned long long wl = __builtin_amdgcn_ballot_w64(lane < ncand && val == top);
+        const int32_t w = __builtin_amdgcn_readlane(ckv, __ffsll((long long)wl) - 1);
+        if (lane == 0) {
+            next[it.v] = w;
+            changed += (w != it.old);
+            if (dist_out) dist_out[it.v] = ncand;
+        }
     }
 };
 
@@ -424,8 +412,9 @@ k_cand(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t
         cand_pipeline<EPT>(pol, nbr, prev, wave, nwaves, hot);
         changed += pol.changed;
     };
-    if (nmt) run(std::integral_constant<int, 16>{}, kCandClasses, cl.nfixed, cl.tile_mt);
-    run(std::integral_constant<int, 16>{}, 3, nt[3], nullptr);
+    constexpr int kTileEPT = kCandTile / kWave;   // slots per lane of a full tile
+    if (nmt) run(std::integral_constant<int, kTileEPT>{}, kCandClasses, cl.nfixed, cl.tile_mt);
+    if (kCandTile > 512) run(std::integral_constant<int, kTileEPT>{}, 3, nt[3], nullptr);
     run(std::integral_constant<int, 8>{}, 2, nt[2], nullptr);
     run(std::integral_constant<int, 4>{}, 1, nt[1], nullptr);
     run(std::integral_constant<int, 2>{}, 0, nt[0], nullptr);
