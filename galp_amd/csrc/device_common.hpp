@@ -82,6 +82,28 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t *scratch) {
     }
 }
 
+// Block-wide exclusive prefix sum of one int per thread (NW waves); `scratch`
+// holds NW entries.  Contains barriers.
+template <int NW>
+__device__ __forceinline__ int block_exclusive_scan(int x, int *scratch) {
+    const int lane = lane_id();
+    const int w = (int)(threadIdx.x / kWave);
+    int inc = x;
+#pragma unroll
+    for (int s = 1; s < kWave; s <<= 1) {
+        const int y = __shfl_up(inc, s);
+        if (lane >= s) inc += y;
+    }
+    if (lane == kWave - 1) scratch[w] = inc;
+    __syncthreads();
+    int before = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+        if (i < w) before += scratch[i];
+    __syncthreads();
+    return before + inc - x;
+}
+
 // Adds a per-thread count into *counter with one atomic per block of NW waves.
 template <int NW>
 __device__ __forceinline__ void block_count_add(uint32_t mine, unsigned long long *counter,

@@ -444,6 +444,8 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     __shared__ uint32_t slist[CAP];
     __shared__ uint32_t nlist;
     __shared__ int32_t slo[T], shi[T];
+    __shared__ int32_t sscan_w[T / kWave];
+    __shared__ int32_t stotal;
     __shared__ unsigned long long sred64[T / kWave];
     if (*stop || (nsel && *nsel == 0)) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
@@ -459,38 +461,49 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
         // tiles in chunks of T: the chunk's slice bounds are loaded one tile
         // per thread (no chain of dependent index loads), then the block
-        // strides over each slice's entries
+        // strides over the chunk's slices concatenated (an exclusive scan of
+        // the slice lengths maps a flat index back to its tile), so every
+        // thread has an entry even when each slice holds a handful
         for (int64_t c0 = t0; c0 < t0 + nt; c0 += T) {
             __syncthreads();
             const int64_t t = c0 + threadIdx.x;
+            int lo = 0, len = 0;
             if (t < t0 + nt) {
                 const int64_t ib = tile_idx_off[t];
-                slo[threadIdx.x] = tile_idx[ib + p];
-                shi[threadIdx.x] = tile_idx[ib + p + 1];
+                lo = tile_idx[ib + p];
+                len = tile_idx[ib + p + 1] - lo;
             }
+            const int start = block_exclusive_scan<T / kWave>(len, sscan_w);   // barriers inside
+            slo[threadIdx.x] = lo;
+            shi[threadIdx.x] = start;   // first flat index of tile c0 + threadIdx.x
+            if (threadIdx.x == T - 1) stotal = start + len;
             __syncthreads();
+            const int total = stotal;
             const int nc = (int)min((int64_t)T, t0 + nt - c0);
-            for (int j = 0; j < nc; ++j) {
-                const int lo = slo[j], hi = shi[j];
-                for (int i0 = lo; i0 < hi; i0 += T) {   // uniform trip count: ballots below
-                    const int i = i0 + (int)threadIdx.x;
-                    bool created = false;
-                    uint32_t sl = 0;
-                    if (i < hi) {
-                        const int64_t e = (c0 + j) * (int64_t)kHubTile + i;
-                        sl = lds_insert<kHubMergeCapBits>(skey, out_key[e], &created, err);
-                        atomicAdd(&scnt[sl], out_cnt[e]);
-                        atomicMax(&slast[sl], out_last[e]);
+            for (int i0 = 0; i0 < total; i0 += T) {   // uniform trip count: ballots below
+                const int i = i0 + (int)threadIdx.x;
+                bool created = false;
+                uint32_t sl = 0;
+                if (i < total) {
+                    int a = 0, b = nc;   // last tile with start <= i
+                    while (b - a > 1) {
+                        const int mid = (a + b) >> 1;
+                        if (shi[mid] <= i) a = mid;
+                        else b = mid;
                     }
-                    // one list append per wave
-                    const unsigned long long ball = __ballot(created);
-                    if (ball) {
-                        const int leader = __ffsll((long long)ball) - 1;
-                        uint32_t base = 0;
-                        if (lane_id() == leader) base = atomicAdd(&nlist, (uint32_t)__popcll(ball));
-                        base = (uint32_t)__shfl((int)base, leader);
-                        if (created) slist[base + __popcll(ball & ((1ull << lane_id()) - 1ull))] = sl;
-                    }
+                    const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - shi[a]);
+                    sl = lds_insert<kHubMergeCapBits>(skey, out_key[e], &created, err);
+                    atomicAdd(&scnt[sl], out_cnt[e]);
+                    atomicMax(&slast[sl], out_last[e]);
+                }
+                // one list append per wave
+                const unsigned long long ball = __ballot(created);
+                if (ball) {
+                    const int leader = __ffsll((long long)ball) - 1;
+                    uint32_t base = 0;
+                    if (lane_id() == leader) base = atomicAdd(&nlist, (uint32_t)__popcll(ball));
+                    base = (uint32_t)__shfl((int)base, leader);
+                    if (created) slist[base + __popcll(ball & ((1ull << lane_id()) - 1ull))] = sl;
                 }
             }
         }
