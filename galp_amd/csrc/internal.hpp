@@ -99,7 +99,12 @@ constexpr int kCtrOverflow = NUM_BUCKETS + kCandClasses;
 constexpr int kCtrCandVerts = kCtrOverflow + 1;
 constexpr int kCtrPlainHubs = kCtrOverflow + 2;           // hubs left to the plain hub path
 constexpr int kCtrCandMt = kCtrOverflow + 3;              // multi-tile vertices routed to candidates
-constexpr int kNumCtrs = kCtrOverflow + 4;
+constexpr int kCtrHd = kCtrOverflow + 4;                  // NUM_BUCKETS sizes of the high-distinct lists
+constexpr int kNumCtrs = kCtrHd + NUM_BUCKETS;
+// A vertex whose predicted distinct-label count exceeds its degree / kHdDiv
+// is "high-distinct": its table kernel inserts lane by lane (no peel rounds,
+// which only pay when a few labels repeat across a chunk).
+constexpr int kHdDiv = 4;
 
 // Candidate path of the learned dispatcher: a wave walks a tile of up to
 // kCandTile edges keeping up to kCandK labels in registers.
@@ -138,9 +143,11 @@ struct LearnState {
     uint32_t *d_pkey = nullptr, *d_pcnt = nullptr, *d_plast = nullptr;
     int32_t *d_hub_sel = nullptr;      // hub left to the plain hub path this iteration
     uint32_t *d_hub_distinct = nullptr;
-    // plain class lists of this iteration (classes W128..B4K)
-    int32_t *plist[NUM_BUCKETS] = {};
+    // plain class lists of this iteration (classes W128..B4K): [c] peeled
+    // inserts, [NUM_BUCKETS + c] high-distinct rows (lane-by-lane inserts)
+    int32_t *plist[2 * NUM_BUCKETS] = {};
     int32_t **d_plist = nullptr;       // device copy of plist
+    int32_t *d_hub_hd = nullptr;       // hub on the plain hub path, predicted high-distinct
     // device counters: [0..NUM_BUCKETS) list sizes, then candidate tiles,
     // candidate overflows, vertices routed to candidates
     int64_t *d_plist_n = nullptr;

@@ -101,11 +101,13 @@ __global__ void k_reset_front(int64_t *ctr, int64_t *cand_bytes) {
 // (Same-address global atomics serialise at the L2; one per wave and list
 // cost ~100 us per iteration on RMAT-22, one per block and list does not.)
 constexpr int kDispChunk = 256;   // one vertex per thread: the MLP is LDS-latency bound
-enum : uint8_t { kCodeCand = NUM_BUCKETS, kCodeCandMt, kCodeHub };
-// LDS list slots: [0, NUM_BUCKETS) class lists, then kCandClasses tile
-// classes, candidate vertices, plain hubs
-constexpr int kSlotTiles = NUM_BUCKETS;
-constexpr int kSlotCandV = NUM_BUCKETS + kCandClasses;
+// codes: [0, NUM_BUCKETS) peeled class list, [NUM_BUCKETS, 2 NUM_BUCKETS)
+// high-distinct class list, then candidate tile, multi-tile candidate, hub
+enum : uint8_t { kCodeCand = 2 * NUM_BUCKETS, kCodeCandMt, kCodeHub };
+// LDS list slots: [0, 2 NUM_BUCKETS) class lists, then kCandClasses tile
+// classes, candidate vertices, plain hubs, multi-tile candidates
+constexpr int kSlotTiles = 2 * NUM_BUCKETS;
+constexpr int kSlotCandV = kSlotTiles + kCandClasses;
 constexpr int kSlotHubs = kSlotCandV + 1;
 constexpr int kSlotMt = kSlotHubs + 1;
 constexpr int kSlots = kSlotMt + 1;
@@ -117,8 +119,9 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
            const float *__restrict__ fmax, const int32_t *__restrict__ dprev, float cand_thresh, int64_t nhubs,
            const int32_t *__restrict__ mt_of, int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel,
            int32_t *__restrict__ cw_v, int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len,
-           int32_t *__restrict__ cw_pslot, const int64_t *__restrict__ cw_base, int32_t *const *__restrict__ plist, int64_t *__restrict__ ctr,
-           int64_t *__restrict__ cand_bytes, const int32_t *__restrict__ stop) {
+           int32_t *__restrict__ cw_pslot, const int64_t *__restrict__ cw_base, int32_t *const *__restrict__ plist,
+           int32_t *__restrict__ hub_hd, int64_t *__restrict__ ctr, int64_t *__restrict__ cand_bytes,
+           const int32_t *__restrict__ stop) {
     __shared__ uint8_t scode[kDispChunk];
     __shared__ int32_t sv[kDispChunk];
     __shared__ int64_t sb[kDispChunk], sd[kDispChunk];
@@ -141,21 +144,25 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
         // weights through the scalar cache: wave-uniform addresses become
         // scalar loads and SGPR operands of the FMAs (no LDS traffic)
         const float y = mlp_forward<H>(params, x[0], x[1], x[2], x[3]);
-        const bool cand = exp2f(y) - 1.f <= cand_thresh;
+        const float dist = exp2f(y) - 1.f;            // predicted distinct labels
+        const bool cand = dist <= cand_thresh;
+        const bool hd = dist * (float)kHdDiv > (float)d;
         const int cls = bucket_of_deg(d);
+        const int list = hd ? NUM_BUCKETS + cls : cls;
         const int mt = d > kCandTile ? mt_of[v] : -1;
         uint8_t code;
         if (mt >= 0) {
             mt_cand[mt] = cand ? 1 : 0;
             if (mt < nhubs) {
                 hub_sel[mt] = cand ? 0 : 1;
+                hub_hd[mt] = (!cand && hd) ? 1 : 0;
                 if (!cand) atomicAdd(&scount[kSlotHubs], 1ull);
             }
-            code = cand ? (uint8_t)kCodeCandMt : cls < B_HUB ? (uint8_t)cls : (uint8_t)kCodeHub;
+            code = cand ? (uint8_t)kCodeCandMt : cls < B_HUB ? (uint8_t)list : (uint8_t)kCodeHub;
         } else {
-            code = cand ? (uint8_t)kCodeCand : (uint8_t)cls;
+            code = cand ? (uint8_t)kCodeCand : (uint8_t)list;
         }
-        if (code < NUM_BUCKETS) atomicAdd(&scount[code], 1ull);
+        if (code < 2 * NUM_BUCKETS) atomicAdd(&scount[code], 1ull);
         if (code == kCodeCand) atomicAdd(&scount[kSlotTiles + cand_class(d)], 1ull);
         if (code == kCodeCandMt) atomicAdd(&scount[kSlotMt], 1ull);
         if (cand) {
@@ -173,6 +180,7 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
     if (threadIdx.x < kSlots) {
         const int c = threadIdx.x;
         const int gc = c < NUM_BUCKETS ? c
+                       : c < 2 * NUM_BUCKETS ? kCtrHd + (c - NUM_BUCKETS)
                        : c < kSlotCandV ? kCtrCandTiles + (c - kSlotTiles)
                        : c == kSlotCandV ? kCtrCandVerts : c == kSlotHubs ? kCtrPlainHubs : kCtrCandMt;
         const unsigned long long cnt = scount[c];
@@ -186,7 +194,7 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
     for (int k = threadIdx.x; k < nloc; k += 256) {
         const uint8_t code = scode[k];
         const int32_t v = sv[k];
-        if (code < NUM_BUCKETS) {
+        if (code < 2 * NUM_BUCKETS) {
             const unsigned long long q = atomicAdd(&scount[code], 1ull);
             plist[code][q] = v;
         } else if (code == kCodeCand) {
@@ -604,10 +612,11 @@ void learn_free(galp_graph *g) {
                     L->d_mt_ctile0, L->d_mt_nctiles, L->d_mt_cand, L->d_pn,     L->d_pkey,     L->d_pcnt,
                     L->d_plast,   L->d_hub_sel,    L->d_hub_distinct, L->d_plist, L->d_plist_n, L->d_log_x,
                     L->d_log_y,   L->d_log_n, L->d_cand_bytes, L->d_cw_base,
-                    L->d_mt_tv,   L->d_mt_te0,     L->d_mt_tlen,  L->d_mt_tslot, L->d_mt_tmt};
+                    L->d_mt_tv,   L->d_mt_te0,     L->d_mt_tlen,  L->d_mt_tslot, L->d_mt_tmt,
+                    L->d_hub_hd};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (int b = 0; b < NUM_BUCKETS; ++b)
+    for (int b = 0; b < 2 * NUM_BUCKETS; ++b)
         if (L->plist[b]) (void)hipFree(L->plist[b]);
     delete L;
     g->learn = nullptr;
@@ -635,8 +644,11 @@ int learn_setup(galp_graph *g) {
     GALP_LTRY(lalloc(g, &L->d_plist_n, kNumCtrs));
     GALP_LTRY(lalloc(g, &L->d_cand_bytes, kMaxIters));
     GALP_LTRY(lalloc(g, &L->d_log_n, 1));
-    for (int b = B_W128; b < B_HUB; ++b) GALP_LTRY(lalloc(g, &L->plist[b], g->bucket[b].nv));
-    GALP_LTRY(lalloc(g, &L->d_plist, NUM_BUCKETS));
+    for (int b = B_W128; b < B_HUB; ++b) {
+        GALP_LTRY(lalloc(g, &L->plist[b], g->bucket[b].nv));
+        GALP_LTRY(lalloc(g, &L->plist[NUM_BUCKETS + b], g->bucket[b].nv));
+    }
+    GALP_LTRY(lalloc(g, &L->d_plist, 2 * NUM_BUCKETS));
     if (hipMemcpy(L->d_plist, L->plist, sizeof(L->plist), hipMemcpyHostToDevice) != hipSuccess)
         return fail(g->ctx, GALP_EHIP, "class list table upload failed");
     // multi-tile vertices: hubs first (g->big_vid lists hubs first)
@@ -712,6 +724,7 @@ int learn_setup(galp_graph *g) {
     GALP_LTRY(lalloc(g, &L->d_pcnt, nct * kCandK));
     GALP_LTRY(lalloc(g, &L->d_plast, nct * kCandK));
     GALP_LTRY(lalloc(g, &L->d_hub_sel, g->nhubs));
+    GALP_LTRY(lalloc(g, &L->d_hub_hd, g->nhubs));
     GALP_LTRY(lalloc(g, &L->d_hub_distinct, g->nhubs));
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -727,6 +740,7 @@ int learn_setup(galp_graph *g) {
     k_fill_i32<<<gd, 256, 0, g->stream>>>((int32_t *)L->d_fmax, n, 0);
     k_fill_i32<<<grid_cap(g, (L->nmt + 255) / 256, 8), 256, 0, g->stream>>>(L->d_mt_cand, L->nmt, 0);
     k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_sel, g->nhubs, 1);
+    k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_hd, g->nhubs, 0);
     k_fill_i32<<<1, 256, 0, g->stream>>>((int32_t *)L->d_hub_distinct, g->nhubs, 0);
     k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_log_n);
     if (big_n) {
@@ -762,7 +776,8 @@ static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl) {
     k_dispatch<H, E><<<(int)((L->big_n + kDispChunk - 1) / kDispChunk), 256, 0, g->stream>>>(
         (const E *)g->d_off, g->d_vlist, L->big_v0, L->big_n, L->d_params, L->d_fdeg, L->d_fmean, L->d_fmax,
         L->d_dist[L->dcur], L->cand_thresh, g->nhubs, L->d_mt_of, L->d_mt_cand, L->d_hub_sel, L->d_cw_v,
-        L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, L->d_cw_base, dpl, L->d_plist_n, L->d_cand_bytes + g->iter % kMaxIters,
+        L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, L->d_cw_base, dpl, L->d_hub_hd, L->d_plist_n,
+        L->d_cand_bytes + g->iter % kMaxIters,
         g->d_stop);
 }
 

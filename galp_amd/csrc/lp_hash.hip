@@ -182,7 +182,7 @@ __device__ __forceinline__ uint32_t lds_insert_chunk(uint32_t *skey, uint32_t *s
 // Pass 2 keeps the largest position whose label has count C: that
 // position's label is argmax (count, last).  Each lane then resets the slots
 // it used, leaving the table empty for the next vertex.
-template <int T, int EPT, int CAPBITS, typename E>
+template <int T, int EPT, int CAPBITS, typename E, int PEEL>
 struct LdsVertexPolicy {
     struct Item {
         bool valid;
@@ -228,7 +228,7 @@ struct LdsVertexPolicy {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * T + (int)threadIdx.x;
-            slot[k] = lds_insert_chunk<CAPBITS, 3, false>(skey, scnt, nullptr, pos < it.len ? lab[k] : -1,
+            slot[k] = lds_insert_chunk<CAPBITS, PEEL, false>(skey, scnt, nullptr, pos < it.len ? lab[k] : -1,
                                                           (uint32_t)pos, &own[k], &creator[k], &maxc, err);
         }
         __syncthreads();
@@ -263,7 +263,9 @@ struct LdsVertexPolicy {
     }
 };
 
-template <int T, int EPT, int CAPBITS, typename E>
+// PEEL: leader-committed peel rounds per 64-edge chunk before the lane-by-
+// lane inserts (3 for rows where a few labels repeat, 0 for high-distinct rows).
+template <int T, int EPT, int CAPBITS, typename E, int PEEL>
 __global__ void __launch_bounds__(T)
 k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
            const int32_t *__restrict__ prev, int32_t *__restrict__ next,
@@ -283,8 +285,8 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
         skey[s] = kEmpty;
         scnt[s] = 0;
     }
-    LdsVertexPolicy<T, EPT, CAPBITS, E> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err,
-                                            distinct, 0u};
+    LdsVertexPolicy<T, EPT, CAPBITS, E, PEEL> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err,
+                                                  distinct, 0u};
     pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
     __syncthreads();
     block_count_add<T / kWave>(pol.changed, counter, sred);
@@ -306,7 +308,9 @@ __device__ __forceinline__ uint32_t hub_partition(uint32_t L, uint32_t P) {
     return (uint32_t)(((unsigned long long)(L * 0x85EBCA77u) * P) >> 32);
 }
 
-template <int T, int EPT, typename E>
+// HD: the launch for high-distinct hubs (lane-by-lane inserts, no peel
+// rounds); with a hub_hd list, each launch takes the hubs of its kind.
+template <int T, int EPT, typename E, bool HD>
 struct HubTilePolicy {
     struct Item {
         bool valid;
@@ -329,6 +333,8 @@ struct HubTilePolicy {
     int32_t *tile_idx;
     int32_t *err;
     const int32_t *hub_sel;   // null: every hub; else only hubs with a non-zero flag
+    int *sscan;               // T / kWave ints of LDS for the block scan
+    const int32_t *hub_hd;    // null: every hub; else only hubs whose flag equals HD
 
     __device__ Item item(int64_t t) const {
         Item it{false, 0, 0, 0, 1, 0, t};
@@ -340,6 +346,7 @@ struct HubTilePolicy {
             it.e0 = rb + it.row0;
             it.len = (int)min((int64_t)kHubTile, re - it.e0);
             if (hub_sel && !hub_sel[h]) it.len = 0;   // processed as a no-op
+            if (hub_hd && (hub_hd[h] != 0) != HD) it.len = 0;   // the other launch's hub
             it.P = (uint32_t)hub_nparts[h];
             it.idx = tile_idx_off[t];
             it.valid = true;
@@ -358,9 +365,9 @@ struct HubTilePolicy {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * T + (int)threadIdx.x;
-            slot[k] = lds_insert_chunk<kHubTileCapBits, 3, true>(skey, scnt, slast, pos < it.len ? lab[k] : -1,
-                                                                 it.row0 + (uint32_t)pos, &own[k], &creator[k],
-                                                                 &maxc, err);
+            slot[k] = lds_insert_chunk<kHubTileCapBits, HD ? 0 : 3, true>(
+                skey, scnt, slast, pos < it.len ? lab[k] : -1, it.row0 + (uint32_t)pos, &own[k], &creator[k], &maxc,
+                err);
         }
         __syncthreads();
         // counting sort of the tile's distinct labels by partition
@@ -372,12 +379,17 @@ struct HubTilePolicy {
                 atomicAdd(&hist[part[k] + 1], 1u);
             }
         __syncthreads();
-        if (threadIdx.x == 0) {   // P_h + 1 offsets; P_h is small next to the tile
-            uint32_t acc = 0;
-            for (uint32_t p = 0; p <= it.P; ++p) {
-                acc += hist[p];
-                hist[p] = acc;
-                tile_idx[it.idx + p] = (int32_t)acc;
+        {   // inclusive scan of the P_h + 1 counts, each thread a run of consecutive ones
+            const uint32_t np = it.P + 1;
+            const uint32_t per = (np + T - 1) / T;
+            const uint32_t p0 = (uint32_t)threadIdx.x * per;
+            int run = 0;
+            for (uint32_t p = p0; p < p0 + per && p < np; ++p) run += (int)hist[p];
+            int acc = block_exclusive_scan<T / kWave>(run, sscan);   // barriers inside
+            for (uint32_t p = p0; p < p0 + per && p < np; ++p) {
+                acc += (int)hist[p];
+                hist[p] = (uint32_t)acc;
+                tile_idx[it.idx + p] = acc;
             }
         }
         __syncthreads();
@@ -401,7 +413,7 @@ struct HubTilePolicy {
     }
 };
 
-template <int T, typename E>
+template <int T, typename E, bool HD>
 __global__ void __launch_bounds__(T)
 k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
            const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
@@ -409,22 +421,24 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
            const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles,
            uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last,
            int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
-           const int32_t *__restrict__ hub_sel, const int64_t *__restrict__ nsel) {
+           const int32_t *__restrict__ hub_sel, const int64_t *__restrict__ nsel,
+           const int32_t *__restrict__ hub_hd) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t slast[CAP];
     __shared__ uint32_t hist[kHubMaxParts + 1];
+    __shared__ int sscan[T / kWave];
     if (*stop || (nsel && *nsel == 0)) return;   // no hub on this path
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
         scnt[s] = 0;
         slast[s] = 0;
     }
-    HubTilePolicy<T, EPT, E> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles,  skey,
+    HubTilePolicy<T, EPT, E, HD> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles,  skey,
                                  scnt, slast,    hist,      out_key,  out_cnt,    out_last,     tile_idx, err,
-                                 hub_sel};
+                                 hub_sel, sscan,   hub_hd};
     pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
@@ -608,12 +622,15 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
     int32_t *dist = g->learn ? g->learn->distinct_out(g) : nullptr;
-    if (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED)   // this iteration's list for the class
-        k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+    if (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) {   // this iteration's lists for the class
+        k_hash_lds<T, EPT, CAPBITS, E, 3><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->learn->plist[bid], 0, r.nv, g->learn->d_plist_n + bid,
             g->d_counter, g->d_stop, g->d_err, dist);
-    else
-        k_hash_lds<T, EPT, CAPBITS, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+        k_hash_lds<T, EPT, CAPBITS, E, 0><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->learn->plist[NUM_BUCKETS + bid], 0, r.nv,
+            g->learn->d_plist_n + kCtrHd + bid, g->d_counter, g->d_stop, g->d_err, dist);
+    } else
+        k_hash_lds<T, EPT, CAPBITS, E, 3><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop,
             g->d_err, dist);
     prof_end(g, &pl);
@@ -693,10 +710,15 @@ static int hash_iteration(galp_graph *g, const IterView &w) {
         PendingLaunch pl;
         prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
         // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
-        k_hub_tile<256, E><<<grid_for(g, nt, 2), 256, 0, g->stream>>>(
+        k_hub_tile<256, E, false><<<grid_for(g, nt, 2), 256, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
             g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
-            g->d_err, sel, nsel);
+            g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr);
+        if (sel)   // learned dispatch: the hubs predicted high-distinct
+            k_hub_tile<256, E, true><<<grid_for(g, nt, 2), 256, 0, g->stream>>>(
+                (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
+                g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+                g->d_err, sel, nsel, g->learn->d_hub_hd);
         prof_end(g, &pl);
         prof_begin(g, "hub_merge", 0.0, &pl);
         k_hub_merge<256><<<grid_for(g, ni, 3), 256, 0, g->stream>>>(
