@@ -14,8 +14,10 @@
 // where its rows fit:
 //   - degree <= 64: a bitonic sort of (label, position) across the group's
 //     lanes (register shuffles), then run ends and run starts from ballots;
-//   - 65..4096: a bitonic sort of (label, position) in LDS per vertex, run
-//     lengths by binary search for each run's first element;
+//   - 65..1024: a bitonic sort of (label, position) in LDS per vertex;
+//     1025..4096: an LDS LSD radix sort of the labels (8-bit digits, stable
+//     multisplit by ballots, positions as payload); run lengths by binary
+//     search for each run's first element;
 //   - hubs (> 4096): a segmented LSD radix sort (8-bit digits, as many passes
 //     as the label width needs) of the hub edges in a compact buffer, then a
 //     per-tile run-length pass; tiles reduce (count, last) with one 64-bit
@@ -76,14 +78,16 @@ struct SortGroupResolve {
 };
 
 // ---------------------------------------------------------------------------
-// 64 < degree <= T*EPT: one block of T threads per vertex.  The (label,
+// Rows of the smaller classes (65..1024): one block of T threads per vertex,
+// a bitonic network in LDS (fewer, cheaper steps than radix passes at these
+// sizes).  The (label,
 // position) keys go to LDS, padded with all-ones to the next power of two,
 // and a bitonic network sorts them; each run end finds its run's first
 // element by binary search (lower bound of its label), giving the count; the
 // block max of (count << 32 | last) picks the winner, whose label the
 // thread that found it writes.
 template <int T, int EPT, typename E>
-struct LdsSortPolicy {
+struct LdsBitonicPolicy {
     static constexpr int N = T * EPT;
     struct Item {
         bool valid;
@@ -178,14 +182,216 @@ struct LdsSortPolicy {
 
 template <int T, int EPT, typename E>
 __global__ void __launch_bounds__(T)
-k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+k_sort_bitonic(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
            int32_t *__restrict__ next, const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
     __shared__ unsigned long long skey[T * EPT];
     __shared__ unsigned long long sred64[T / kWave];
     __shared__ uint32_t sred[T / kWave];
     if (*stop) return;
-    LdsSortPolicy<T, EPT, E> pol{off, prev, next, vlist, v0, nv, skey, sred64, err, 0u};
+    LdsBitonicPolicy<T, EPT, E> pol{off, prev, next, vlist, v0, nv, skey, sred64, err, 0u};
+    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    __syncthreads();
+    block_count_add<T / kWave>(pol.changed, counter, sred);
+}
+
+// ---------------------------------------------------------------------------
+// Rows of B2K and B4K (1025..4096): one block of T threads per vertex, an LSD radix sort
+// of the row's labels in LDS (8-bit digits, `passes` of them), each pass a
+// stable multisplit: wave w owns the contiguous elements [w*M, (w+1)*M) (M =
+// N / waves) and walks them in 64-element chunks in order; 8 ballots give
+// each lane the lanes of its chunk with the same digit, whose popcount
+// below it is its rank; per-wave digit histograms, one scan over (digit,
+// wave) and per-wave running bases make the pass stable across the block.
+// Positions ride along as the payload, so equal labels stay in row order.
+// Then each run end finds its run's first element by binary search; the
+// block max of (count << 32 | last position) picks the winner.
+// Lanes of this wave whose 8-bit digit equals this lane's, among `valid`.
+__device__ __forceinline__ unsigned long long digit_peers8(uint32_t d, bool valid) {
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    return peers;
+}
+
+template <int T, int EPT, typename E>
+struct LdsSortPolicy {
+    static constexpr int N = T * EPT;
+    static constexpr int NW = T / kWave;
+    static constexpr int M = N / NW;        // elements per wave
+    static constexpr int CH = M / kWave;    // 64-element chunks per wave
+    struct Item {
+        bool valid;
+        int len;
+        int64_t v;
+        int64_t b;
+        int32_t old;
+    };
+    const E *off;
+    const int32_t *prev;
+    int32_t *next;
+    const int32_t *vlist;
+    int64_t v0, nv;
+    uint32_t *skey;          // 2 x N
+    uint16_t *spos;          // 2 x N
+    uint32_t *whist;         // NW x 256 (counts, then running bases)
+    unsigned long long *sred64;
+    int32_t *err;
+    int passes;
+    uint32_t changed;
+
+    __device__ Item item(int64_t i) const {
+        Item it{false, 0, 0, 0, 0};
+        if (i < nv) {
+            it.v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
+            it.b = (int64_t)off[it.v];
+            it.len = (int)((int64_t)off[it.v + 1] - it.b);
+            it.old = prev[it.v];
+            it.valid = true;
+            if (it.len > N) {   // never planned into this class
+                if (threadIdx.x == 0) atomicOr(err, 8);
+                it.len = 0;
+            }
+        }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.b; }
+
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        if (it.len == 0) return;
+        const int len = it.len;
+        const int tid = (int)threadIdx.x;
+        const int lane = lane_id();
+        const int w = tid / kWave;
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        __syncthreads();   // the previous vertex's readers are done
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + tid;
+            if (pos < len) {
+                skey[pos] = (uint32_t)lab[k];
+                spos[pos] = (uint16_t)pos;
+            }
+        }
+        int src = 0;
+        for (int p = 0; p < passes; ++p) {
+            const int shift = 8 * p;
+            const uint32_t *ks = skey + src * N;
+            const uint16_t *ps = spos + src * N;
+            uint32_t *kd = skey + (src ^ 1) * N;
+            uint16_t *pd = spos + (src ^ 1) * N;
+            for (int i = lane; i < 256; i += kWave) whist[w * 256 + i] = 0;
+            __syncthreads();   // keys written; histogram rows clear
+            // A: per-wave digit counts (the lowest lane of each peer group adds)
+            for (int c = 0; c < CH; ++c) {
+                const int j = w * M + c * kWave + lane;
+                const bool valid = j < len;
+                const uint32_t d = valid ? (ks[j] >> shift) & 0xffu : 0u;
+                const unsigned long long peers = digit_peers8(d, valid);
+                if (valid && (peers & lt) == 0) whist[w * 256 + d] += (uint32_t)__popcll(peers);
+                if (w * M + c * kWave >= len) break;   // uniform per wave
+            }
+            __syncthreads();
+            // bases: digit-major exclusive scan over (digit, wave)
+            if (w == 0) {
+                uint32_t tot[4];
+                uint32_t run = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {   // lane owns digits 4*lane .. 4*lane+3
+                    uint32_t t = 0;
+                    for (int ww = 0; ww < NW; ++ww) t += whist[ww * 256 + 4 * lane + q];
+                    tot[q] = t;
+                    run += t;
+                }
+                uint32_t inc = run;
+#pragma unroll
+                for (int sft = 1; sft < kWave; sft <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
+                    if (lane >= sft) inc += y;
+                }
+                uint32_t base = inc - run;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int d = 4 * lane + q;
+                    uint32_t b2 = base;
+                    for (int ww = 0; ww < NW; ++ww) {
+                        const uint32_t cnt = whist[ww * 256 + d];
+                        whist[ww * 256 + d] = b2;
+                        b2 += cnt;
+                    }
+                    base += tot[q];
+                }
+            }
+            __syncthreads();
+            // B: stable scatter, each wave in chunk order with its running bases
+            for (int c = 0; c < CH; ++c) {
+                if (w * M + c * kWave >= len) break;   // uniform per wave
+                const int j = w * M + c * kWave + lane;
+                const bool valid = j < len;
+                const uint32_t key = valid ? ks[j] : 0u;
+                const uint32_t d = (key >> shift) & 0xffu;
+                const unsigned long long peers = digit_peers8(d, valid);
+                const uint32_t base = valid ? whist[w * 256 + d] : 0u;
+                if (valid) {
+                    const uint32_t dst = base + (uint32_t)__popcll(peers & lt);
+                    kd[dst] = key;
+                    pd[dst] = ps[j];
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (valid && (peers & lt) == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
+                __builtin_amdgcn_wave_barrier();
+            }
+            src ^= 1;
+            __syncthreads();
+        }
+        const uint32_t *kf = skey + src * N;
+        const uint16_t *pf = spos + src * N;
+        unsigned long long best = 0;
+        int32_t bestlab = 0;
+        for (int q = tid; q < len; q += T) {
+            const uint32_t key = kf[q];
+            const bool end = q == len - 1 || kf[q + 1] != key;
+            if (end) {
+                int lo = 0, hi = q;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (kf[mid] < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                const unsigned long long val = ((unsigned long long)(q - lo + 1) << 32) | (uint32_t)pf[q];
+                if (val > best) {
+                    best = val;
+                    bestlab = (int32_t)key;
+                }
+            }
+        }
+        const unsigned long long top = block_max<T / kWave>(best, sred64);
+        if (best == top) {   // unique: positions are distinct
+            next[it.v] = bestlab;
+            changed += (bestlab != it.old);
+        }
+    }
+};
+
+template <int T, int EPT, typename E>
+__global__ void __launch_bounds__(T)
+k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+           int32_t *__restrict__ next, const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+           unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
+           int passes) {
+    static_assert(T * EPT <= 65536, "positions are 16-bit");
+    static_assert((T * EPT / (T / kWave)) % kWave == 0, "whole chunks per wave");
+    __shared__ uint32_t skey[2 * T * EPT];
+    __shared__ uint16_t spos[2 * T * EPT];
+    __shared__ uint32_t whist[(T / kWave) * 256];
+    __shared__ unsigned long long sred64[T / kWave];
+    __shared__ uint32_t sred[T / kWave];
+    if (*stop) return;
+    LdsSortPolicy<T, EPT, E> pol{off, prev, next, vlist, v0, nv, skey, spos, whist, sred64, err, passes, 0u};
     pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
     __syncthreads();
     block_count_add<T / kWave>(pol.changed, counter, sred);
@@ -656,15 +862,21 @@ static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t 
     prof_end(g, &pl);
 }
 
-template <int T, int EPT, typename E>
+// RADIX: the LSD radix sort (large rows) or the bitonic network (smaller).
+template <int T, int EPT, typename E, bool RADIX>
 static void sort_lds(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
                      int blocks_per_cu) {
     const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
-    k_sort_lds<T, EPT, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-        (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
+    if constexpr (RADIX)
+        k_sort_lds<T, EPT, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err,
+            g->sort->passes);
+    else
+        k_sort_bitonic<T, EPT, E><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
     prof_end(g, &pl);
 }
 
@@ -726,14 +938,14 @@ static int sort_iteration(galp_graph *g, const IterView &w) {
         // "load-imbalanced" ablation: every class <= 4096 through the
         // 512-thread block-per-vertex kernel, as the reference's li path
         // runs one block per vertex whatever its degree.
-        for (int b = B_B4K; b >= B_G4; --b) sort_lds<512, 8, E>(g, w, b, prev, next, 2);
+        for (int b = B_B4K; b >= B_G4; --b) sort_lds<512, 8, E, true>(g, w, b, prev, next, 2);
     } else {
-        sort_lds<512, 8, E>(g, w, B_B4K, prev, next, 2);
-        sort_lds<256, 8, E>(g, w, B_B2K, prev, next, 4);
-        sort_lds<128, 8, E>(g, w, B_B1K, prev, next, 8);
-        sort_lds<64, 8, E>(g, w, B_W512, prev, next, 16);
-        sort_lds<64, 4, E>(g, w, B_W256, prev, next, 24);
-        sort_lds<64, 2, E>(g, w, B_W128, prev, next, 32);
+        sort_lds<512, 8, E, true>(g, w, B_B4K, prev, next, 2);
+        sort_lds<256, 8, E, true>(g, w, B_B2K, prev, next, 4);
+        sort_lds<128, 8, E, false>(g, w, B_B1K, prev, next, 8);
+        sort_lds<64, 8, E, false>(g, w, B_W512, prev, next, 16);
+        sort_lds<64, 4, E, false>(g, w, B_W256, prev, next, 24);
+        sort_lds<64, 2, E, false>(g, w, B_W128, prev, next, 32);
         sort_group<64, E>(g, w, B_G64, prev, next);
         sort_group<32, E>(g, w, B_G32, prev, next);
         sort_group<16, E>(g, w, B_G16, prev, next);
