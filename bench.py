@@ -237,6 +237,10 @@ def main():
     table = {"hash_learned": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_LEARNED),
              "hash_plain": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_PLAIN),
              "hash_tuned": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_PLAIN, tune=True),
+             "hash_learned_tuned_graph": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_LEARNED, tune=True,
+                                              graph_capture=1),
+             "hash_tuned_graph": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_PLAIN, tune=True,
+                                      graph_capture=1),
              "hash_learned_tuned": dict(method=_abi.METHOD_HASH, dispatch=_abi.DISPATCH_LEARNED, tune=True),
              "sort": dict(method=_abi.METHOD_SORT, dispatch=_abi.DISPATCH_PLAIN)}
     variants = {}

@@ -263,13 +263,17 @@ struct LdsVertexPolicy {
     }
 };
 
-// PEEL: leader-committed peel rounds per 64-edge chunk before the lane-by-
-// lane inserts (3 for rows where a few labels repeat, 0 for high-distinct rows).
-template <int T, int EPT, int CAPBITS, typename E, int PEEL>
+// Two vertex lists in one launch: list A with PEEL_A leader-committed peel
+// rounds per 64-edge chunk before the lane-by-lane inserts, then list B with
+// none (the high-distinct rows of learned dispatch; nvB = 0 otherwise).
+// Peel rounds pay where a few labels repeat across a chunk; each phase is its
+// own pipeline, so no per-row branch sits between the loads in flight.
+template <int T, int EPT, int CAPBITS, typename E, int PEEL_A>
 __global__ void __launch_bounds__(T)
 k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
            const int32_t *__restrict__ prev, int32_t *__restrict__ next,
            const int32_t *__restrict__ vlist, int64_t v0, int64_t nv, const int64_t *__restrict__ nv_dev,
+           const int32_t *__restrict__ vlistB, const int64_t *__restrict__ nvB_dev,
            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
            int32_t *__restrict__ err, int32_t *__restrict__ distinct) {
     static_assert((1 << CAPBITS) >= 2 * T * EPT, "load factor must stay <= 1/2");
@@ -279,17 +283,29 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
     __shared__ uint32_t sred[T / kWave];
     __shared__ unsigned long long sred64[T / kWave];
     if (*stop) return;
-    if (nv_dev) nv = *nv_dev;   // list built on the device this iteration
-    if ((int64_t)blockIdx.x >= nv) return;
+    if (nv_dev) nv = *nv_dev;   // lists built on the device this iteration
+    const int64_t nvB = nvB_dev ? *nvB_dev : 0;
+    if ((int64_t)blockIdx.x >= nv && (int64_t)blockIdx.x >= nvB) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
         scnt[s] = 0;
     }
-    LdsVertexPolicy<T, EPT, CAPBITS, E, PEEL> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64, err,
-                                                  distinct, 0u};
-    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    uint32_t changed = 0;
+    if ((int64_t)blockIdx.x < nv) {   // uniform per block
+        LdsVertexPolicy<T, EPT, CAPBITS, E, PEEL_A> pol{off, prev, next, vlist, v0, nv, skey, scnt, sred, sred64,
+                                                        err, distinct, 0u};
+        pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+        changed += pol.changed;
+    }
+    if ((int64_t)blockIdx.x < nvB) {
+        __syncthreads();
+        LdsVertexPolicy<T, EPT, CAPBITS, E, 0> pol{off, prev, next, vlistB, 0, nvB, skey, scnt, sred, sred64, err,
+                                                   distinct, 0u};
+        pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+        changed += pol.changed;
+    }
     __syncthreads();
-    block_count_add<T / kWave>(pol.changed, counter, sred);
+    block_count_add<T / kWave>(changed, counter, sred);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,17 +638,15 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
     int32_t *dist = g->learn ? g->learn->distinct_out(g) : nullptr;
-    if (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) {   // this iteration's lists for the class
+    if (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED)   // this iteration's two lists for the class
         k_hash_lds<T, EPT, CAPBITS, E, 3><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->learn->plist[bid], 0, r.nv, g->learn->d_plist_n + bid,
-            g->d_counter, g->d_stop, g->d_err, dist);
-        k_hash_lds<T, EPT, CAPBITS, E, 0><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, g->learn->plist[NUM_BUCKETS + bid], 0, r.nv,
-            g->learn->d_plist_n + kCtrHd + bid, g->d_counter, g->d_stop, g->d_err, dist);
-    } else
-        k_hash_lds<T, EPT, CAPBITS, E, 3><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop,
+            g->learn->plist[NUM_BUCKETS + bid], g->learn->d_plist_n + kCtrHd + bid, g->d_counter, g->d_stop,
             g->d_err, dist);
+    else
+        k_hash_lds<T, EPT, CAPBITS, E, 3><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, nullptr, nullptr, g->d_counter,
+            g->d_stop, g->d_err, dist);
     prof_end(g, &pl);
 }
 
