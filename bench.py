@@ -42,7 +42,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS per LP iteration + % HBM peak; learned vs plain dispatch, bit-exact labels"
 MLP_HIDDEN = 32
-CAND_THRESH = 10.0
+CAND_THRESH = 14.0
 
 
 def log(*a):

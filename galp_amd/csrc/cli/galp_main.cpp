@@ -161,7 +161,7 @@ int main(int argc, char **argv) {
         float loss = 0.f;
         if (rc == GALP_OK && nsamp > 0)
             rc = galp_mlp_train(ctx, hidden, params.data(), x.data(), y.data(), nsamp, 8, 512, 3e-3f, 2, &loss);
-        if (rc == GALP_OK) rc = galp_graph_set_mlp(dg, hidden, params.data(), 10.0f);
+        if (rc == GALP_OK) rc = galp_graph_set_mlp(dg, hidden, params.data(), 14.0f);
         if (rc != GALP_OK) {
             std::printf("galp: %s\n", galp_last_error(ctx));
             return 1;

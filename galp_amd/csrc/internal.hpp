@@ -104,7 +104,7 @@ constexpr int kNumCtrs = kCtrHd + NUM_BUCKETS;
 // A vertex whose predicted distinct-label count exceeds its degree / kHdDiv
 // is "high-distinct": its table kernel inserts lane by lane (no peel rounds,
 // which only pay when a few labels repeat across a chunk).
-constexpr int kHdDiv = 4;
+constexpr int kHdDiv = 8;
 
 // Candidate path of the learned dispatcher: a wave walks a tile of up to
 // kCandTile edges keeping up to kCandK labels in registers.
