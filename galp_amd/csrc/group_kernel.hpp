@@ -13,13 +13,9 @@ namespace galp {
 // while round t is resolved, the labels of round t+1 and the neighbour ids
 // of round t+2 are loading, and the offsets of round t+3.
 template <int G, class Resolve, bool VLIST, typename E>
-__global__ void __launch_bounds__(256)
-k_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
-             const int32_t *__restrict__ prev, int32_t *__restrict__ next,
-             const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
-             unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
-    __shared__ uint32_t scratch[256 / kWave];
-    if (*stop) return;
+__device__ __forceinline__ uint32_t group_rounds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
+                                                 const int32_t *__restrict__ prev, int32_t *__restrict__ next,
+                                                 const int32_t *__restrict__ vlist, int64_t v0, int64_t nv) {
     const int p = lane_id() & (G - 1);
     constexpr int64_t per_block = 256 / G;
     const int64_t stride = (int64_t)gridDim.x * per_block;
@@ -82,6 +78,24 @@ k_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
         GALP_GROUP_STAGE(sd, sa, sb, sc, LB, oB, LA, oA)
     }
 #undef GALP_GROUP_STAGE
+    return changed;
+}
+
+// Alt: the resolver of iterations where most rows are high-distinct (the
+// learned dispatcher's count of such big vertices, *hd_count, exceeds half of
+// hd_total); chosen once per launch, so each choice is its own pipeline.
+template <int G, class Resolve, class Alt, bool VLIST, typename E>
+__global__ void __launch_bounds__(256)
+k_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
+        const int32_t *__restrict__ prev, int32_t *__restrict__ next,
+        const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+        unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
+        const int64_t *__restrict__ hd_count, int64_t hd_total) {
+    __shared__ uint32_t scratch[256 / kWave];
+    if (*stop) return;
+    const bool alt = hd_count && 2 * (*hd_count) > hd_total;   // uniform
+    const uint32_t changed = alt ? group_rounds<G, Alt, VLIST, E>(off, nbr, prev, next, vlist, v0, nv)
+                                 : group_rounds<G, Resolve, VLIST, E>(off, nbr, prev, next, vlist, v0, nv);
     block_count_add<4>(changed, counter, scratch);
 }
 

@@ -100,7 +100,8 @@ constexpr int kCtrCandVerts = kCtrOverflow + 1;
 constexpr int kCtrPlainHubs = kCtrOverflow + 2;           // hubs left to the plain hub path
 constexpr int kCtrCandMt = kCtrOverflow + 3;              // multi-tile vertices routed to candidates
 constexpr int kCtrHd = kCtrOverflow + 4;                  // NUM_BUCKETS sizes of the high-distinct lists
-constexpr int kNumCtrs = kCtrHd + NUM_BUCKETS;
+constexpr int kCtrHdAll = kCtrHd + NUM_BUCKETS;           // big vertices predicted high-distinct (any path)
+constexpr int kNumCtrs = kCtrHdAll + 1;
 // A vertex whose predicted distinct-label count exceeds its degree / kHdDiv
 // is "high-distinct": its table kernel inserts lane by lane (no peel rounds,
 // which only pay when a few labels repeat across a chunk).

@@ -110,7 +110,8 @@ constexpr int kSlotTiles = 2 * NUM_BUCKETS;
 constexpr int kSlotCandV = kSlotTiles + kCandClasses;
 constexpr int kSlotHubs = kSlotCandV + 1;
 constexpr int kSlotMt = kSlotHubs + 1;
-constexpr int kSlots = kSlotMt + 1;
+constexpr int kSlotHdAll = kSlotMt + 1;
+constexpr int kSlots = kSlotHdAll + 1;
 
 template <int H, typename E>
 __global__ void __launch_bounds__(256)
@@ -163,6 +164,7 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
             code = cand ? (uint8_t)kCodeCand : (uint8_t)list;
         }
         if (code < 2 * NUM_BUCKETS) atomicAdd(&scount[code], 1ull);
+        if (hd) atomicAdd(&scount[kSlotHdAll], 1ull);
         if (code == kCodeCand) atomicAdd(&scount[kSlotTiles + cand_class(d)], 1ull);
         if (code == kCodeCandMt) atomicAdd(&scount[kSlotMt], 1ull);
         if (cand) {
@@ -182,7 +184,8 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
         const int gc = c < NUM_BUCKETS ? c
                        : c < 2 * NUM_BUCKETS ? kCtrHd + (c - NUM_BUCKETS)
                        : c < kSlotCandV ? kCtrCandTiles + (c - kSlotTiles)
-                       : c == kSlotCandV ? kCtrCandVerts : c == kSlotHubs ? kCtrPlainHubs : kCtrCandMt;
+                       : c == kSlotCandV ? kCtrCandVerts : c == kSlotHubs ? kCtrPlainHubs
+                       : c == kSlotMt ? kCtrCandMt : kCtrHdAll;
         const unsigned long long cnt = scount[c];
         scount[c] = cnt ? atomicAdd((unsigned long long *)(ctr + gc), cnt) : 0ull;
         if (c >= kSlotTiles && c < kSlotCandV) scount[c] += (unsigned long long)cw_base[c - kSlotTiles];

@@ -98,6 +98,12 @@ struct HashGroupResolve {
     template <int G>
     __device__ __forceinline__ static uint32_t best(int32_t L) { return group_best<G, (G < 4 ? G : 4)>(L); }
 };
+// High-distinct iterations: the pairwise sweep alone (peel rounds resolve one
+// lane each when every label is rare).
+struct HashGroupSweep {
+    template <int G>
+    __device__ __forceinline__ static uint32_t best(int32_t L) { return group_best<G, 0>(L); }
+};
 
 // Open-addressing insert into an LDS table of 2^BITS slots; returns the slot
 // and sets *created when this lane claimed an empty slot.  The caller
@@ -621,12 +627,18 @@ static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
     const int64_t per_block = 256 / G;
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
+    // learned dispatch: the sweep-only resolver in iterations where most big
+    // vertices are predicted high-distinct
+    const bool learned = g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED;
+    const int64_t *hd = learned ? g->learn->d_plist_n + kCtrHdAll : nullptr;
+    const int64_t hd_total = learned ? g->learn->big_n : 0;
     if (g->d_vlist)
-        k_group<G, HashGroupResolve, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
+        k_group<G, HashGroupResolve, HashGroupSweep, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, hd, hd_total);
     else
-        k_group<G, HashGroupResolve, false, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
+        k_group<G, HashGroupResolve, HashGroupSweep, false, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0,
+                                                                 g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop, hd, hd_total);
     prof_end(g, &pl);
 }
 

@@ -854,11 +854,11 @@ static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t 
     const int64_t per_block = 256 / G;
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
     if (g->d_vlist)
-        k_group<G, SortGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop);
+        k_group<G, SortGroupResolve, SortGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, nullptr, 0);
     else
-        k_group<G, SortGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop);
+        k_group<G, SortGroupResolve, SortGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop, nullptr, 0);
     prof_end(g, &pl);
 }
 
