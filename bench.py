@@ -184,7 +184,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--graph", choices=("rmat", "lfr"), default="rmat",
+                    help="rmat: RMAT --scale; lfr: LFR-style graph of --lfr-n vertices (gamma 2.5, mean degree 20, "
+                         "max degree 2000, community sizes 50..50000 with exponent 1.5, mu 0.3, seed 8)")
     ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--lfr-n", type=int, default=8_000_000)
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--variants", default="hash_learned_tuned,hash_tuned,hash_learned,hash_plain,sort")
     ap.add_argument("--batch-graphs", type=int, default=512,
@@ -214,9 +218,17 @@ def main():
         return
 
     t = time.perf_counter()
-    off, nbr = _abi.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=args.scale)
+    if args.graph == "rmat":
+        off, nbr = _abi.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=args.scale)
+        gname = f"RMAT-{args.scale}"
+        workload = f"RMAT scale {args.scale}, edge factor 16, a=0.57 b=0.19 c=0.19, seed {args.scale}"
+    else:
+        off, nbr = _abi.gen_lfr(args.lfr_n, 2.5, 20.0, 2000, 1.5, 50, 50000, 0.3, seed=8)
+        gname = f"LFR-{args.lfr_n}"
+        workload = (f"LFR-style graph, n={args.lfr_n}, degree exponent 2.5, mean degree 20, max degree 2000, "
+                    f"community sizes 50..50000 (exponent 1.5), mu=0.3, seed 8")
     n, m = len(off) - 1, len(nbr)
-    log(f"bench: RMAT-{args.scale} n={n} m={m} generated in {time.perf_counter() - t:.1f}s")
+    log(f"bench: {gname} n={n} m={m} generated in {time.perf_counter() - t:.1f}s")
 
     ctx = _abi.Context(0)
     dev = ctx.device_info()
@@ -283,8 +295,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"RMAT scale {args.scale}, edge factor 16, a=0.57 b=0.19 c=0.19, seed {args.scale}, "
-                               f"{args.iters} synchronous LP iterations per step, in-core",
+        "config": {"workload": f"{workload}, {args.iters} synchronous LP iterations per step, in-core",
                    "n": n, "m": m, "method": "hash (method 1)" if head != "sort" else "sort (method 0)",
                    "dispatch": "learned" if head.startswith("hash_learned") else "plain",
                    "autotuned": head.endswith("_tuned"),
@@ -321,7 +332,7 @@ def main():
         k = len(counts)
         out["cpu_baseline"] = {"value": round(m * k / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": cores,
                                "kind": "port",
-                               "sample": f"the full workload: RMAT-{args.scale}, {k} iterations, "
+                               "sample": f"the full workload: {gname}, {k} iterations, "
                                          f"{cpu_s:.2f} s wall on {cores} threads"}
         out["labels_bit_exact"] = {name: bool(np.array_equal(v["labels"], final)) for name, v in variants.items()}
     print(json.dumps(out), flush=True)

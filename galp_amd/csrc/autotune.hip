@@ -2,8 +2,9 @@
 //
 // No reference counterpart (the reference fixes its block sizes at compile
 // time, lfht.cuh:127-165).  For every class of the hash method with
-// vertices, each of its configurations (launch_class in lp_hash.hip:
-// workgroup size x slots per lane and blocks per CU) is launched `reps`
+// vertices, hubs included, each of its configurations (launch_class in
+// lp_hash.hip: workgroup size x slots per lane and blocks per CU; for hubs
+// the tile and merge workgroup sizes) is launched `reps`
 // times on the graph's current labels and timed with HIP events; the
 // fastest is kept.  The choice never changes labels (every configuration
 // computes the same rule), which is how it is tested.  Choices can be
@@ -80,7 +81,7 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
     const int32_t dispatch = g->opts.dispatch;
     g->prof.on = false;
     g->opts.dispatch = GALP_DISPATCH_PLAIN;   // time each class over all of its vertices
-    for (int b = 0; b < B_HUB && rc == GALP_OK; ++b) {
+    for (int b = 0; b <= B_HUB && rc == GALP_OK; ++b) {
         const BucketRange &r = g->bucket[b];
         if (!r.nv) continue;
         const int lv = ilog2(r.nv), le = ilog2(r.edges);

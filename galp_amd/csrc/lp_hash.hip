@@ -464,7 +464,9 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
     pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
-template <int T>
+// LIST: keep a list of the slots created (argmax and reset visit only them);
+// without it they scan the whole table, which frees 16 KB of LDS per block.
+template <int T, bool LIST>
 __global__ void __launch_bounds__(T)
 k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t nitems,
             const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
@@ -477,7 +479,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t slast[CAP];
-    __shared__ uint32_t slist[CAP];
+    __shared__ uint32_t slist[LIST ? CAP : 1];
     __shared__ uint32_t nlist;
     __shared__ int32_t slo[T], shi[T];
     __shared__ int32_t sscan_w[T / kWave];
@@ -533,7 +535,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
                     atomicMax(&slast[sl], out_last[e]);
                 }
                 // one list append per wave
-                const unsigned long long ball = __ballot(created);
+                const unsigned long long ball = LIST ? __ballot(created) : 0ull;
                 if (ball) {
                     const int leader = __ffsll((long long)ball) - 1;
                     uint32_t base = 0;
@@ -546,19 +548,35 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         __syncthreads();
         // argmax and reset over the slots created (cost follows the number of
         // distinct labels, not the table size)
-        const uint32_t nl = nlist;
         unsigned long long best = 0;
-        for (uint32_t i = threadIdx.x; i < nl; i += T) {
-            const uint32_t sl = slist[i];
-            const unsigned long long pk = ((unsigned long long)scnt[sl] << 32) | slast[sl];
-            best = pk > best ? pk : best;
-            skey[sl] = kEmpty;
-            scnt[sl] = 0;
-            slast[sl] = 0;
+        uint32_t nl = 0;
+        if constexpr (LIST) {
+            nl = nlist;
+            for (uint32_t i = threadIdx.x; i < nl; i += T) {
+                const uint32_t sl = slist[i];
+                const unsigned long long pk = ((unsigned long long)scnt[sl] << 32) | slast[sl];
+                best = pk > best ? pk : best;
+                skey[sl] = kEmpty;
+                scnt[sl] = 0;
+                slast[sl] = 0;
+            }
+        } else {
+            for (int sl = threadIdx.x; sl < CAP; sl += T)
+                if (skey[sl] != kEmpty) {
+                    const unsigned long long pk = ((unsigned long long)scnt[sl] << 32) | slast[sl];
+                    best = pk > best ? pk : best;
+                    ++nl;
+                    skey[sl] = kEmpty;
+                    scnt[sl] = 0;
+                    slast[sl] = 0;
+                }
         }
         __syncthreads();
         if (threadIdx.x == 0) nlist = 0;
         best = block_max<T / kWave>(best, sred64);
+        if constexpr (!LIST) {
+            if (hub_distinct) nl = block_sum<T / kWave>(nl, (uint32_t *)sscan_w);   // uniform branch
+        }
         if (threadIdx.x == 0 && best) atomicMax(&hub_best[h], best);
         if (threadIdx.x == 0 && hub_distinct) atomicAdd(&hub_distinct[h], nl);
     }
@@ -662,14 +680,60 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
     prof_end(g, &pl);
 }
 
+// Hub rows: tile (TT threads per 2048-edge tile), merge (TM threads per
+// (hub, partition) item), final.  The tile holds 64 KB of LDS per block and
+// the merge 68 KB with its slot list, 52 KB without (two or three blocks per
+// CU), so block size and list set the waves per CU.
+template <typename E, int TT, int TM, bool LIST>
+static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next) {
+    if (w.h1 <= w.h0) return;
+    const int64_t nh = w.h1 - w.h0, nt = w.t1 - w.t0, ni = w.i1 - w.i0;
+    double hub_edges = 0.0;
+    if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
+    const int32_t *sel = (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) ? g->learn->d_hub_sel : nullptr;
+    uint32_t *hdist = g->learn ? g->learn->d_hub_distinct : nullptr;
+    const int64_t *nsel = sel ? g->learn->d_plist_n + kCtrPlainHubs : nullptr;
+    PendingLaunch pl;
+    prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
+    // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
+    k_hub_tile<TT, E, false><<<grid_for(g, nt, 2), TT, 0, g->stream>>>(
+        (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
+        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+        g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr);
+    if (sel)   // learned dispatch: the hubs predicted high-distinct
+        k_hub_tile<TT, E, true><<<grid_for(g, nt, 2), TT, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
+            g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+            g->d_err, sel, nsel, g->learn->d_hub_hd);
+    prof_end(g, &pl);
+    prof_begin(g, "hub_merge", 0.0, &pl);
+    k_hub_merge<TM, LIST><<<grid_for(g, ni, TM == 256 || !LIST ? 3 : 2), TM, 0, g->stream>>>(
+        g->d_item_hub + w.i0, g->d_item_part + w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
+        g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, sel, hdist,
+        nsel);
+    prof_end(g, &pl);
+    prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);
+    k_hub_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
+        (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, sel ? sel + w.h0 : nullptr,
+        hdist ? hdist + w.h0 : nullptr, g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter,
+        g->d_stop, nsel);
+    prof_end(g, &pl);
+}
+
 // The kernel configuration of each degree class: option 0 is the default,
 // 1 and 2 the alternatives the autotuner times (workgroup size x slots per
 // lane, and blocks per CU for the grid; for lane-group classes the grid
-// only).  Every option computes the same labels.
+// only; for hubs the tile and merge workgroup sizes).  Every option computes
+// the same labels.
 template <typename E>
 static void launch_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
     const int c = g->tune[b];
     switch (b) {
+    case B_HUB:
+        if (c == 1) launch_hubs<E, 512, 512, true>(g, w, prev, next);
+        else if (c == 2) launch_hubs<E, 256, 256, true>(g, w, prev, next);
+        else launch_hubs<E, 512, 512, false>(g, w, prev, next);
+        break;
     case B_B4K:
         if (c == 1) launch_lds<1024, 4, 13, E>(g, w, b, prev, next, 1);
         else if (c == 2) launch_lds<256, 16, 13, E>(g, w, b, prev, next, 4);
@@ -725,40 +789,7 @@ template <typename E>
 static int hash_iteration(galp_graph *g, const IterView &w) {
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
-    // largest work first
-    if (w.h1 > w.h0) {
-        const int64_t nh = w.h1 - w.h0, nt = w.t1 - w.t0, ni = w.i1 - w.i0;
-        double hub_edges = 0.0;
-        if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
-        const int32_t *sel = (g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED) ? g->learn->d_hub_sel : nullptr;
-        uint32_t *hdist = g->learn ? g->learn->d_hub_distinct : nullptr;
-        const int64_t *nsel = sel ? g->learn->d_plist_n + kCtrPlainHubs : nullptr;
-        PendingLaunch pl;
-        prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
-        // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
-        k_hub_tile<256, E, false><<<grid_for(g, nt, 2), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-            g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
-            g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr);
-        if (sel)   // learned dispatch: the hubs predicted high-distinct
-            k_hub_tile<256, E, true><<<grid_for(g, nt, 2), 256, 0, g->stream>>>(
-                (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-                g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
-                g->d_err, sel, nsel, g->learn->d_hub_hd);
-        prof_end(g, &pl);
-        prof_begin(g, "hub_merge", 0.0, &pl);
-        k_hub_merge<256><<<grid_for(g, ni, 3), 256, 0, g->stream>>>(
-            g->d_item_hub + w.i0, g->d_item_part + w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
-            g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, sel,
-            hdist, nsel);
-        prof_end(g, &pl);
-        prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);
-        k_hub_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, sel ? sel + w.h0 : nullptr,
-            hdist ? hdist + w.h0 : nullptr, g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter,
-            g->d_stop, nsel);
-        prof_end(g, &pl);
-    }
+    launch_class<E>(g, w, B_HUB, prev, next);   // largest work first
     if (g->opts.variant == 0) {
         // -l 0: no degree classes below the hubs -- every row through the
         // block-per-vertex table kernel, as the reference's naive variant
