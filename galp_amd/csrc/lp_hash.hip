@@ -377,13 +377,33 @@ struct HubTilePolicy {
     }
     __device__ int64_t edge0(const Item &it) const { return it.e0; }
 
+    // inclusive scan of the P + 1 partition counts in hist, each thread a run
+    // of consecutive ones; hist[p] becomes the end of partition p - 1, and the
+    // scanned counts are the tile's partition offsets
+    __device__ void scan_parts(const Item &it) {
+        const uint32_t np = it.P + 1;
+        const uint32_t per = (np + T - 1) / T;
+        const uint32_t p0 = (uint32_t)threadIdx.x * per;
+        int run = 0;
+        for (uint32_t p = p0; p < p0 + per && p < np; ++p) run += (int)hist[p];
+        int acc = block_exclusive_scan<T / kWave>(run, sscan);   // barriers inside
+        for (uint32_t p = p0; p < p0 + per && p < np; ++p) {
+            acc += (int)hist[p];
+            hist[p] = (uint32_t)acc;
+            tile_idx[it.idx + p] = acc;
+        }
+    }
+
+    // The table is clean (keys empty, counts and positions 0) when a tile
+    // starts and again when it ends.  In between, the tile's distinct labels
+    // move to registers and the table is reset, so the count array serves as
+    // the partition histogram (counts are 0 after the reset, so it starts
+    // cleared); it is zeroed again at the end.
     __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
         if (it.len == 0) return;   // hub left to the candidate path: the merge skips it too
         uint32_t slot[EPT];
         bool own[EPT], creator[EPT];
         uint32_t maxc = 0;
-        for (uint32_t p = threadIdx.x; p <= it.P; p += T) hist[p] = 0;
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * T + (int)threadIdx.x;
@@ -392,37 +412,14 @@ struct HubTilePolicy {
                 err);
         }
         __syncthreads();
-        // counting sort of the tile's distinct labels by partition
-        uint32_t part[EPT];
+        uint32_t key[EPT], cnt[EPT], last[EPT], part[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k)
             if (creator[k]) {
-                part[k] = hub_partition(skey[slot[k]], it.P);
-                atomicAdd(&hist[part[k] + 1], 1u);
-            }
-        __syncthreads();
-        {   // inclusive scan of the P_h + 1 counts, each thread a run of consecutive ones
-            const uint32_t np = it.P + 1;
-            const uint32_t per = (np + T - 1) / T;
-            const uint32_t p0 = (uint32_t)threadIdx.x * per;
-            int run = 0;
-            for (uint32_t p = p0; p < p0 + per && p < np; ++p) run += (int)hist[p];
-            int acc = block_exclusive_scan<T / kWave>(run, sscan);   // barriers inside
-            for (uint32_t p = p0; p < p0 + per && p < np; ++p) {
-                acc += (int)hist[p];
-                hist[p] = (uint32_t)acc;
-                tile_idx[it.idx + p] = acc;
-            }
-        }
-        __syncthreads();
-        const int64_t ob = it.t * (int64_t)kHubTile;
-#pragma unroll
-        for (int k = 0; k < EPT; ++k)
-            if (creator[k]) {
-                const uint32_t q = atomicAdd(&hist[part[k]], 1u);
-                out_key[ob + q] = skey[slot[k]];
-                out_cnt[ob + q] = scnt[slot[k]];
-                out_last[ob + q] = slast[slot[k]];
+                key[k] = skey[slot[k]];
+                cnt[k] = scnt[slot[k]];
+                last[k] = slast[slot[k]];
+                part[k] = hub_partition(key[k], it.P);
             }
         __syncthreads();
 #pragma unroll
@@ -432,6 +429,26 @@ struct HubTilePolicy {
                 scnt[slot[k]] = 0;
                 slast[slot[k]] = 0;
             }
+        __syncthreads();
+        // counting sort of the tile's distinct labels by partition
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+            if (creator[k]) atomicAdd(&hist[part[k] + 1], 1u);
+        __syncthreads();
+        scan_parts(it);
+        __syncthreads();
+        const int64_t ob = it.t * (int64_t)kHubTile;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+            if (creator[k]) {
+                const uint32_t q = atomicAdd(&hist[part[k]], 1u);
+                out_key[ob + q] = key[k];
+                out_cnt[ob + q] = cnt[k];
+                out_last[ob + q] = last[k];
+            }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p <= it.P; p += T) hist[p] = 0;
+        __syncthreads();
     }
 };
 
@@ -447,20 +464,22 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
            const int32_t *__restrict__ hub_hd) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
-    __shared__ uint32_t skey[CAP];
-    __shared__ uint32_t scnt[CAP];
-    __shared__ uint32_t slast[CAP];
-    __shared__ uint32_t hist[kHubMaxParts + 1];
+    // keys | counts | positions; the partition histogram (P + 1 entries)
+    // reuses the counts and, for P = kHubMaxParts, the first position
+    static_assert(kHubMaxParts + 1 <= 2 * CAP, "partition histogram fits in the table");
+    __shared__ uint32_t tab[3 * CAP];
     __shared__ int sscan[T / kWave];
     if (*stop || (nsel && *nsel == 0)) return;   // no hub on this path
     for (int s = threadIdx.x; s < CAP; s += T) {
-        skey[s] = kEmpty;
-        scnt[s] = 0;
-        slast[s] = 0;
+        tab[s] = kEmpty;
+        tab[CAP + s] = 0;
+        tab[2 * CAP + s] = 0;
     }
-    HubTilePolicy<T, EPT, E, HD> pol{off,  tile_hub, hub_tile0, hub_vid,  hub_nparts, tile_idx_off, ntiles,  skey,
-                                 scnt, slast,    hist,      out_key,  out_cnt,    out_last,     tile_idx, err,
-                                 hub_sel, sscan,   hub_hd};
+    __syncthreads();
+    HubTilePolicy<T, EPT, E, HD> pol{off,      tile_hub, hub_tile0,     hub_vid,  hub_nparts, tile_idx_off,
+                                     ntiles,   tab,      tab + CAP,     tab + 2 * CAP,        tab + CAP,
+                                     out_key,  out_cnt,  out_last,      tile_idx, err,        hub_sel,
+                                     sscan,    hub_hd};
     pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
@@ -681,8 +700,8 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
 }
 
 // Hub rows: tile (TT threads per 2048-edge tile), merge (TM threads per
-// (hub, partition) item), final.  The tile holds 64 KB of LDS per block and
-// the merge 68 KB with its slot list, 52 KB without (two or three blocks per
+// (hub, partition) item), final.  The tile holds 48 KB of LDS per block and
+// the merge 68 KB with its slot list, 52 KB without (three or two blocks per
 // CU), so block size and list set the waves per CU.
 template <typename E, int TT, int TM, bool LIST>
 static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next) {
@@ -696,12 +715,12 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
     PendingLaunch pl;
     prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
     // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
-    k_hub_tile<TT, E, false><<<grid_for(g, nt, 2), TT, 0, g->stream>>>(
+    k_hub_tile<TT, E, false><<<grid_for(g, nt, 3), TT, 0, g->stream>>>(
         (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
         g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
         g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr);
     if (sel)   // learned dispatch: the hubs predicted high-distinct
-        k_hub_tile<TT, E, true><<<grid_for(g, nt, 2), TT, 0, g->stream>>>(
+        k_hub_tile<TT, E, true><<<grid_for(g, nt, 3), TT, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
             g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
             g->d_err, sel, nsel, g->learn->d_hub_hd);
