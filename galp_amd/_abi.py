@@ -113,6 +113,8 @@ def lib():
                                                       ctypes.POINTER(ctypes.POINTER(HostCsr))]),
         "galp_gen_rmat": (ctypes.c_int, [i32, i32, dbl, dbl, dbl, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.POINTER(HostCsr))]),
+        "galp_gen_batch_planted": (ctypes.c_int, [i64, i64, i64, i64, i64, i32, i32, dbl, ctypes.c_uint64,
+                                                  ctypes.POINTER(ctypes.POINTER(HostCsr)), vp, vp]),
         "galp_gen_lfr": (ctypes.c_int, [i64, dbl, dbl, i32, dbl, i32, i32, dbl, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.POINTER(HostCsr)),
                                         ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))]),
@@ -165,6 +167,21 @@ def gen_planted_partition(n, community, k_intra, k_inter, seed):
     p = ctypes.POINTER(HostCsr)()
     _check_host(lib().galp_gen_planted_partition(n, community, k_intra, k_inter, seed, ctypes.byref(p)))
     return _take_csr(p)
+
+
+def gen_batch_planted(count, n_range=(1000, 200000), comm_range=(32, 1024), deg_range=(8, 48), inter_frac=0.25,
+                      seed=4096):
+    """`count` planted-partition graphs as one disjoint-union CSR.  Returns
+    (off, nbr, vbase, spec): graph k owns vertices [vbase[k], vbase[k+1]) and
+    is gen_planted_partition(*spec[k]) shifted by vbase[k]."""
+    p = ctypes.POINTER(HostCsr)()
+    vbase = np.zeros(count + 1, np.int64)
+    spec = np.zeros((count, 5), np.int64)
+    _check_host(lib().galp_gen_batch_planted(count, n_range[0], n_range[1], comm_range[0], comm_range[1],
+                                             deg_range[0], deg_range[1], inter_frac, seed, ctypes.byref(p),
+                                             vbase.ctypes.data, spec.ctypes.data))
+    off, nbr = _take_csr(p)
+    return off, nbr, vbase, spec
 
 
 def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):

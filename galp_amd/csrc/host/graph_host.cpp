@@ -188,41 +188,144 @@ static int load_bin(const std::string &path, galp_host_csr **out) {
 }
 
 // -------------------------------------------------------- generators ----
+// The draws of vertex v of a planted-partition graph: kin partners from its
+// own community (without v), kout from the other communities; emit(v, w)
+// and emit(w, v) for each.
+template <class Emit>
+static inline void pp_vertex(int64_t v, int64_t n, int64_t cs, int64_t ncomm, int32_t kin, int32_t kout,
+                             uint64_t seed, Emit &&emit) {
+    SplitMix64 r(seed, (uint64_t)v);
+    const int64_t c = v / cs, base = c * cs;
+    const int64_t size = std::min(cs, n - base);
+    for (int32_t i = 0; i < kin; ++i) {
+        int64_t w = v;
+        if (size > 1) {
+            int64_t x = (int64_t)r.below((uint64_t)(size - 1));
+            if (x >= v - base) ++x;
+            w = base + x;
+        }
+        emit(v, w);
+    }
+    for (int32_t i = 0; i < kout; ++i) {
+        int64_t w = v;
+        if (ncomm > 1) {
+            int64_t c2 = (int64_t)r.below((uint64_t)(ncomm - 1));
+            if (c2 >= c) ++c2;
+            int64_t b2 = c2 * cs, s2 = std::min(cs, n - b2);
+            w = b2 + (int64_t)r.below((uint64_t)s2);
+        }
+        emit(v, w);
+    }
+}
+
+static bool pp_params_ok(int64_t n, int64_t cs, int32_t kin, int32_t kout) {
+    return n > 0 && n <= INT32_MAX && cs > 0 && kin >= 0 && kout >= 0;
+}
+
 static int gen_pp(int64_t n, int64_t cs, int32_t kin, int32_t kout, uint64_t seed,
                   galp_host_csr **out) {
-    if (n <= 0 || n > INT32_MAX || cs <= 0 || kin < 0 || kout < 0)
-        return host_fail(GALP_EINVAL, "planted partition: bad parameters");
+    if (!pp_params_ok(n, cs, kin, kout)) return host_fail(GALP_EINVAL, "planted partition: bad parameters");
     const int64_t ncomm = (n + cs - 1) / cs;
     const int64_t per = (int64_t)kin + kout;
     std::vector<uint32_t> src((size_t)(2 * n * per)), dst((size_t)(2 * n * per));
     parallel_for(0, n, [&](int64_t v) {
-        SplitMix64 r(seed, (uint64_t)v);
-        int64_t c = v / cs, base = c * cs;
-        int64_t size = std::min(cs, n - base);
         size_t k = (size_t)(2 * v * per);
-        for (int32_t i = 0; i < kin; ++i) {
-            int64_t w = v;
-            if (size > 1) {
-                int64_t x = (int64_t)r.below((uint64_t)(size - 1));
-                if (x >= v - base) ++x;
-                w = base + x;
-            }
-            src[k] = (uint32_t)v; dst[k] = (uint32_t)w; ++k;
-            src[k] = (uint32_t)w; dst[k] = (uint32_t)v; ++k;
-        }
-        for (int32_t i = 0; i < kout; ++i) {
-            int64_t w = v;
-            if (ncomm > 1) {
-                int64_t c2 = (int64_t)r.below((uint64_t)(ncomm - 1));
-                if (c2 >= c) ++c2;
-                int64_t b2 = c2 * cs, s2 = std::min(cs, n - b2);
-                w = b2 + (int64_t)r.below((uint64_t)s2);
-            }
-            src[k] = (uint32_t)v; dst[k] = (uint32_t)w; ++k;
-            src[k] = (uint32_t)w; dst[k] = (uint32_t)v; ++k;
-        }
+        pp_vertex(v, n, cs, ncomm, kin, kout, seed, [&](int64_t a, int64_t b) {
+            src[k] = (uint32_t)a; dst[k] = (uint32_t)b; ++k;
+            src[k] = (uint32_t)b; dst[k] = (uint32_t)a; ++k;
+        });
     });
     return csr_from_pairs(n, src, dst, out);
+}
+
+// One planted-partition graph built on the calling thread (the batch
+// generator runs one graph per worker): the same graph as gen_pp, since rows
+// are sorted and deduplicated whatever the order the pairs come in.
+static void pp_serial(int64_t n, int64_t cs, int32_t kin, int32_t kout, uint64_t seed, std::vector<int64_t> &off,
+                      std::vector<int32_t> &nbr) {
+    const int64_t ncomm = (n + cs - 1) / cs;
+    const int64_t per = (int64_t)kin + kout;
+    std::vector<int32_t> src((size_t)(2 * n * per)), dst((size_t)(2 * n * per));
+    size_t k = 0;
+    for (int64_t v = 0; v < n; ++v)
+        pp_vertex(v, n, cs, ncomm, kin, kout, seed, [&](int64_t a, int64_t b) {
+            src[k] = (int32_t)a; dst[k] = (int32_t)b; ++k;
+            src[k] = (int32_t)b; dst[k] = (int32_t)a; ++k;
+        });
+    std::vector<int64_t> cur((size_t)n + 1, 0);
+    for (size_t i = 0; i < k; ++i)
+        if (src[i] != dst[i]) cur[(size_t)src[i] + 1]++;
+    for (int64_t v = 0; v < n; ++v) cur[(size_t)v + 1] += cur[(size_t)v];
+    std::vector<int32_t> tmp((size_t)std::max<int64_t>(cur[(size_t)n], 1));
+    std::vector<int64_t> pos(cur.begin(), cur.end() - 1);
+    for (size_t i = 0; i < k; ++i)
+        if (src[i] != dst[i]) tmp[(size_t)pos[(size_t)src[i]]++] = dst[i];
+    std::vector<int32_t>().swap(src);
+    std::vector<int32_t>().swap(dst);
+    off.assign((size_t)n + 1, 0);
+    nbr.clear();
+    nbr.reserve((size_t)cur[(size_t)n]);
+    for (int64_t v = 0; v < n; ++v) {
+        int32_t *b = tmp.data() + cur[(size_t)v], *e = tmp.data() + cur[(size_t)v + 1];
+        std::sort(b, e);
+        e = std::unique(b, e);
+        nbr.insert(nbr.end(), b, e);
+        off[(size_t)v + 1] = (int64_t)nbr.size();
+    }
+    nbr.shrink_to_fit();
+}
+
+// Graph k of a batch: its size, community size, draws and seed, all from the
+// stream (seed, k).
+static void batch_spec(uint64_t seed, int64_t k, int64_t n_lo, int64_t n_hi, int64_t c_lo, int64_t c_hi, int32_t d_lo,
+                       int32_t d_hi, double inter_frac, int64_t spec[5]) {
+    SplitMix64 r(seed, (uint64_t)k);
+    const int64_t n = n_lo + (int64_t)r.below((uint64_t)(n_hi - n_lo + 1));
+    const int64_t cs = c_lo + (int64_t)r.below((uint64_t)(c_hi - c_lo + 1));
+    const int64_t d = d_lo + (int64_t)r.below((uint64_t)(d_hi - d_lo + 1));
+    spec[0] = n;
+    spec[1] = cs;
+    spec[2] = (d + 1) / 2;                                                        // draws: degree ~ d
+    spec[3] = std::max<int64_t>(1, std::llround(inter_frac * (double)d / 2.0));   // degree ~ inter_frac * d
+    spec[4] = (int64_t)(mix64(seed ^ 0x5851f42d4c957f2dULL) + (uint64_t)k);
+}
+
+static int gen_batch_pp(int64_t count, int64_t n_lo, int64_t n_hi, int64_t c_lo, int64_t c_hi, int32_t d_lo,
+                        int32_t d_hi, double inter_frac, uint64_t seed, galp_host_csr **out, int64_t *vbase,
+                        int64_t *spec_out) {
+    if (count < 1 || n_lo < 1 || n_hi < n_lo || c_lo < 1 || c_hi < c_lo || d_lo < 0 || d_hi < d_lo ||
+        !(inter_frac >= 0.0))
+        return host_fail(GALP_EINVAL, "batch planted partition: bad parameters");
+    std::vector<int64_t> spec((size_t)(5 * count));
+    vbase[0] = 0;
+    for (int64_t k = 0; k < count; ++k) {
+        batch_spec(seed, k, n_lo, n_hi, c_lo, c_hi, d_lo, d_hi, inter_frac, &spec[(size_t)(5 * k)]);
+        vbase[k + 1] = vbase[k] + spec[(size_t)(5 * k)];
+    }
+    if (vbase[count] > INT32_MAX) return host_fail(GALP_EINVAL, "batch: more than 2^31 - 1 vertices in all");
+    std::vector<std::vector<int64_t>> offs((size_t)count);
+    std::vector<std::vector<int32_t>> nbrs((size_t)count);
+    parallel_for_dynamic(0, count, 1, [&](int64_t k) {
+        const int64_t *sp = &spec[(size_t)(5 * k)];
+        pp_serial(sp[0], sp[1], (int32_t)sp[2], (int32_t)sp[3], (uint64_t)sp[4], offs[(size_t)k], nbrs[(size_t)k]);
+    });
+    std::vector<int64_t> ebase((size_t)count + 1, 0);
+    for (int64_t k = 0; k < count; ++k) ebase[(size_t)k + 1] = ebase[(size_t)k] + (int64_t)nbrs[(size_t)k].size();
+    galp_host_csr *g = csr_alloc(vbase[count], ebase[(size_t)count]);
+    if (!g) return host_fail(GALP_ENOMEM, "host allocation failed for the batch CSR");
+    g->offsets[0] = 0;
+    parallel_for_dynamic(0, count, 1, [&](int64_t k) {
+        const std::vector<int64_t> &o = offs[(size_t)k];
+        const int64_t vb = vbase[k], eb = ebase[(size_t)k];
+        for (size_t v = 1; v < o.size(); ++v) g->offsets[vb + (int64_t)v] = eb + o[v];
+        int32_t *dst = g->neighbors + eb;
+        for (int32_t u : nbrs[(size_t)k]) *dst++ = u + (int32_t)vb;
+        std::vector<int64_t>().swap(offs[(size_t)k]);
+        std::vector<int32_t>().swap(nbrs[(size_t)k]);
+    });
+    if (spec_out) std::memcpy(spec_out, spec.data(), sizeof(int64_t) * spec.size());
+    *out = g;
+    return GALP_OK;
 }
 
 static std::vector<uint32_t> fisher_yates(int64_t n, uint64_t seed) {
@@ -458,6 +561,18 @@ GALP_API int galp_gen_planted_partition(int64_t n, int64_t community_size, int32
         return gen_pp(n, community_size, k_intra, k_inter, seed, out);
     } catch (const std::bad_alloc &) {
         return host_fail(GALP_ENOMEM, "host allocation failed (planted partition)");
+    }
+}
+
+GALP_API int galp_gen_batch_planted(int64_t count, int64_t n_lo, int64_t n_hi, int64_t comm_lo, int64_t comm_hi,
+                                    int32_t deg_lo, int32_t deg_hi, double inter_frac, uint64_t seed,
+                                    galp_host_csr **out, int64_t *vbase, int64_t *spec) {
+    if (!out || !vbase) return host_fail(GALP_EINVAL, "null out or vbase");
+    *out = nullptr;
+    try {
+        return gen_batch_pp(count, n_lo, n_hi, comm_lo, comm_hi, deg_lo, deg_hi, inter_frac, seed, out, vbase, spec);
+    } catch (const std::bad_alloc &) {
+        return host_fail(GALP_ENOMEM, "host allocation failed (batch planted partition)");
     }
 }
 

@@ -61,6 +61,23 @@ GALP_API int galp_gen_planted_partition(int64_t n, int64_t community_size,
                                         int32_t k_intra, int32_t k_inter,
                                         uint64_t seed, galp_host_csr **out);
 
+/* A batch of `count` independent planted-partition graphs as one CSR, their
+ * disjoint union: graph k owns vertices [vbase[k], vbase[k+1]).  Graph k's
+ * parameters come from the SplitMix64 stream (seed, k): n ~ U[n_lo, n_hi],
+ * community size ~ U[comm_lo, comm_hi], degree target d ~ U[deg_lo, deg_hi];
+ * it is galp_gen_planted_partition(n, community, (d + 1) / 2,
+ * max(1, round(inter_frac * d / 2)), seed_k), so intra degree ~ d and inter
+ * degree ~ inter_frac * d after symmetrisation.  vbase holds count + 1
+ * entries; spec (optional) receives count rows of (n, community, k_intra,
+ * k_inter, seed_k).  Graphs are built in parallel, one per host thread.
+ * No reference counterpart (the reference runs one graph per process). */
+GALP_API int galp_gen_batch_planted(int64_t count, int64_t n_lo, int64_t n_hi,
+                                    int64_t comm_lo, int64_t comm_hi,
+                                    int32_t deg_lo, int32_t deg_hi,
+                                    double inter_frac, uint64_t seed,
+                                    galp_host_csr **out, int64_t *vbase,
+                                    int64_t *spec);
+
 /* R-MAT / Kronecker graph (Chakrabarti et al. 2004; Graph500 parameters
  * without noise): edge_factor * 2^scale samples with quadrant probabilities
  * a, b, c, d = 1-a-b-c; self-loops dropped, symmetrised, deduplicated, vertex

@@ -94,6 +94,27 @@ def test_planted_partition_shape_and_determinism():
     assert not np.array_equal(off, off3)
 
 
+def test_batch_generator_is_the_disjoint_union():
+    # every slice is the single-graph generator's output for its spec row
+    count = 24
+    off, nbr, vbase, spec = _abi.gen_batch_planted(count, n_range=(500, 3000), comm_range=(32, 256),
+                                                   deg_range=(8, 48), inter_frac=0.25, seed=7)
+    assert vbase[0] == 0 and vbase[-1] == len(off) - 1 and np.all(np.diff(vbase) == spec[:, 0])
+    check_simple_symmetric(off, nbr)
+    for k in range(count):
+        n, cs, kin, kout, sd = (int(x) for x in spec[k])
+        assert 500 <= n <= 3000 and 32 <= cs <= 256 and 4 <= kin <= 24 and kout >= 1
+        o, nb = _abi.gen_planted_partition(n, cs, kin, kout, sd & ((1 << 64) - 1))
+        a, b = int(vbase[k]), int(vbase[k + 1])
+        assert np.array_equal(off[a:b + 1] - off[a], o), k
+        assert np.array_equal(nbr[off[a]:off[b]] - a, nb), k   # no edge leaves its graph
+    # degree ~ d + d/4 with d ~ U[8, 48] (35 on average), less the repeated
+    # draws that small communities cause
+    assert 18 < len(nbr) / (len(off) - 1) < 42
+    off2, nbr2, _, _ = _abi.gen_batch_planted(count, n_range=(500, 3000), comm_range=(32, 256), seed=7)
+    assert np.array_equal(off, off2) and np.array_equal(nbr, nbr2)
+
+
 def test_rmat_shape():
     off, nbr = _abi.gen_rmat(12, 16, seed=5)
     n = 4096
