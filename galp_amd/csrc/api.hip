@@ -473,7 +473,7 @@ static int launch_iteration(galp_graph *g) {
         if (g->opts.dispatch == GALP_DISPATCH_LEARNED) {
             if (!g->learn || !g->learn->H)
                 return fail(g->ctx, GALP_ESTATE, "learned dispatch needs a network: call galp_graph_set_mlp");
-            GALP_TRY(launch_learned_front(g));
+            if (!g->ooc) GALP_TRY(launch_learned_front(g));   // out of core: per batch
         }
         if (g->ooc) GALP_TRY(launch_ooc_iteration(g));
         else GALP_TRY(launch_hash_iteration(g, full_view(g)));
@@ -616,8 +616,9 @@ GALP_API int galp_graph_upload(galp_ctx *ctx, const galp_csr_view *csr, const ga
     if ((o.policy == GALP_POLICY_OOC_SYNC || o.policy == GALP_POLICY_OOC_ASYNC) &&
         (o.buffer_pow < 10 || o.buffer_pow > 34))
         return fail(ctx, GALP_EINVAL, "buffer_pow (-b) must be in [10, 34] for the out-of-core policies");
-    if ((o.policy == GALP_POLICY_OOC_SYNC || o.policy == GALP_POLICY_OOC_ASYNC) && o.dispatch == GALP_DISPATCH_LEARNED)
-        return fail(ctx, GALP_EUNSUPPORTED, "learned dispatch is in-core only");
+    if ((o.policy == GALP_POLICY_OOC_SYNC || o.policy == GALP_POLICY_OOC_ASYNC) && o.dispatch == GALP_DISPATCH_LEARNED &&
+        !o.reorder)
+        return fail(ctx, GALP_EUNSUPPORTED, "learned dispatch out-of-core needs the degree renumbering (reorder)");
     if (o.dispatch == GALP_DISPATCH_LEARNED && (o.method != GALP_METHOD_HASH || o.variant != 1))
         return fail(ctx, GALP_EUNSUPPORTED, "learned dispatch routes method 1, -l 1");
     if (o.variant < 0 || o.variant > 3) return fail(ctx, GALP_EINVAL, "variant (-l) must be 0..3");

@@ -125,6 +125,7 @@ struct LearnState {
     int32_t *d_dist[2] = {nullptr, nullptr};   // distinct labels per vertex, double-buffered
     int dcur = 0;                      // d_dist[dcur]: previous iteration's counts
     int64_t big_v0 = 0, big_n = 0;     // big vertex i: vlist ? vlist[big_v0+i] : big_v0+i
+    int64_t front_n = 0;               // big vertices the last dispatch routed (an out-of-core batch's share)
     float cand_thresh = 4.0f;          // predicted distinct labels routed to candidates
     // candidate work list (tiles)
     int64_t cw_cap = 0;
@@ -299,6 +300,8 @@ int ooc_setup(galp_graph *g, const galp_csr_view *c, const std::vector<int64_t> 
               const std::vector<int32_t> &hub_tile0, const std::vector<int64_t> &hub_item0);
 void ooc_free(galp_graph *g);
 int launch_ooc_iteration(galp_graph *g);
+// Streams every batch through learn_features_view once (learned dispatch set-up).
+int ooc_features(galp_graph *g);
 
 // Drops the captured iteration loop after a change to what it launches.
 inline void invalidate_captured(galp_graph *g) {
@@ -317,6 +320,11 @@ int learn_setup(galp_graph *g);
 void learn_free(galp_graph *g);
 int learn_reset_iteration(galp_graph *g);
 int launch_learned_front(galp_graph *g);
+// Out-of-core: the dispatch and candidate path of one batch (vertices
+// [lo, hi), the batch's rebased offsets and neighbour buffer in w).
+int launch_learned_front_batch(galp_graph *g, const IterView &w, int64_t lo, int64_t hi);
+// Static features of the big vertices among [lo, hi) from a batch's rows.
+int learn_features_view(galp_graph *g, const void *offrow, const int32_t *nbr, int64_t lo, int64_t hi);
 int learn_after_iteration(galp_graph *g);
 
 // Graph bookkeeping kernels (graph_dev.hip), all on g->stream.

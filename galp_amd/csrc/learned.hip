@@ -43,16 +43,18 @@ __device__ __forceinline__ void mlp_inputs(float deg, float mean, float mx, int3
 // ---------------------------------------------------------------------------
 // Static features: degree, mean and max neighbour degree (one wave per big
 // vertex, once at upload).  mean = (float)((double)sum / deg), the exact
-// integer sum rounded once.
+// integer sum rounded once.  offrow locates the rows in nbr (an out-of-core
+// batch's rebased offsets), off gives every vertex's degree.
 template <typename E>
 __global__ void __launch_bounds__(256)
-k_features(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ vlist,
-           int64_t v0, int64_t nv, float *__restrict__ fdeg, float *__restrict__ fmean, float *__restrict__ fmax) {
+k_features(const E *__restrict__ offrow, const int32_t *__restrict__ nbr, const E *__restrict__ off,
+           const int32_t *__restrict__ vlist, int64_t v0, int64_t nv, float *__restrict__ fdeg,
+           float *__restrict__ fmean, float *__restrict__ fmax) {
     const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
     const int64_t nw = (int64_t)gridDim.x * (256 / kWave);
     for (int64_t i = wave; i < nv; i += nw) {
         const int64_t v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
-        const int64_t b = (int64_t)off[v], d = (int64_t)off[v + 1] - b;
+        const int64_t b = (int64_t)offrow[v], d = (int64_t)offrow[v + 1] - b;
         unsigned long long sum = 0, mx = 0;
         for (int64_t j = lane_id(); j < d; j += kWave) {
             const int64_t u = nbr[b + j];
@@ -90,9 +92,13 @@ __global__ void k_reset_ctrs(int64_t *ctr) {
     if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
 }
 
-__global__ void k_reset_front(int64_t *ctr, int64_t *cand_bytes) {
-    if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
-    if (threadIdx.x == 0) *cand_bytes = 0;
+// Clears the dispatch's lists and counters.  first == 0 (out-of-core batches
+// after the first of an iteration): the iteration's statistics -- overflows,
+// vertices routed to candidates, candidate-path bytes -- keep adding up.
+__global__ void k_reset_front(int64_t *ctr, int64_t *cand_bytes, int first) {
+    const int c = threadIdx.x;
+    if (c < kNumCtrs && (first || (c != kCtrOverflow && c != kCtrCandVerts))) ctr[c] = 0;
+    if (c == 0 && first) *cand_bytes = 0;
 }
 
 // One block routes kDispChunk consecutive big vertices.  Appends are
@@ -116,7 +122,7 @@ constexpr int kSlots = kSlotHdAll + 1;
 template <int H, typename E>
 __global__ void __launch_bounds__(256)
 k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t big_v0, int64_t big_n,
-           const float *__restrict__ params, const float *__restrict__ fdeg, const float *__restrict__ fmean,
+           bool allow_mt, const float *__restrict__ params, const float *__restrict__ fdeg, const float *__restrict__ fmean,
            const float *__restrict__ fmax, const int32_t *__restrict__ dprev, float cand_thresh, int64_t nhubs,
            const int32_t *__restrict__ mt_of, int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel,
            int32_t *__restrict__ cw_v, int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len,
@@ -146,7 +152,9 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
         // scalar loads and SGPR operands of the FMAs (no LDS traffic)
         const float y = mlp_forward<H>(params, x[0], x[1], x[2], x[3]);
         const float dist = exp2f(y) - 1.f;            // predicted distinct labels
-        const bool cand = dist <= cand_thresh;
+        // rows of several tiles use the fixed tile list, which holds in-core
+        // edge offsets: out of core they stay on the class and hub paths
+        const bool cand = dist <= cand_thresh && (allow_mt || d <= kCandTile);
         const bool hd = dist * (float)kHdDiv > (float)d;
         const int cls = bucket_of_deg(d);
         const int list = hd ? NUM_BUCKETS + cls : cls;
@@ -628,7 +636,8 @@ void learn_free(galp_graph *g) {
 // Allocates the learned-dispatch state and computes the static features.
 int learn_setup(galp_graph *g) {
     if (g->learn) return GALP_OK;
-    if (g->ooc) return fail(g->ctx, GALP_EUNSUPPORTED, "the learned dispatcher's features need the graph in-core");
+    if (g->ooc && !g->reorder)   // a batch's big vertices are then one id range
+        return fail(g->ctx, GALP_EUNSUPPORTED, "learned dispatch out-of-core needs the degree renumbering (reorder)");
     LearnState *L = new (std::nothrow) LearnState();
     if (!L) return fail(g->ctx, GALP_ENOMEM, "host allocation failed");
     g->learn = L;
@@ -746,18 +755,36 @@ int learn_setup(galp_graph *g) {
     k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_hd, g->nhubs, 0);
     k_fill_i32<<<1, 256, 0, g->stream>>>((int32_t *)L->d_hub_distinct, g->nhubs, 0);
     k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_log_n);
-    if (big_n) {
+    if (big_n && g->ooc) {
+        GALP_LTRY(ooc_features(g));   // the rows stream through the batch buffers
+    } else if (big_n) {
         const int gf = grid_cap(g, (big_n + 3) / 4, 64);
         if (g->off64)
-            k_features<int64_t><<<gf, 256, 0, g->stream>>>((const int64_t *)g->d_off, g->d_nbr, g->d_vlist,
-                                                           L->big_v0, big_n, L->d_fdeg, L->d_fmean, L->d_fmax);
+            k_features<int64_t><<<gf, 256, 0, g->stream>>>((const int64_t *)g->d_off, g->d_nbr,
+                                                           (const int64_t *)g->d_off, g->d_vlist, L->big_v0, big_n,
+                                                           L->d_fdeg, L->d_fmean, L->d_fmax);
         else
-            k_features<int32_t><<<gf, 256, 0, g->stream>>>((const int32_t *)g->d_off, g->d_nbr, g->d_vlist,
-                                                           L->big_v0, big_n, L->d_fdeg, L->d_fmean, L->d_fmax);
+            k_features<int32_t><<<gf, 256, 0, g->stream>>>((const int32_t *)g->d_off, g->d_nbr,
+                                                           (const int32_t *)g->d_off, g->d_vlist, L->big_v0, big_n,
+                                                           L->d_fdeg, L->d_fmean, L->d_fmax);
     }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess)
         return fail(g->ctx, GALP_EHIP, "feature kernels failed");
     return GALP_OK;
+}
+
+int learn_features_view(galp_graph *g, const void *offrow, const int32_t *nbr, int64_t lo, int64_t hi) {
+    LearnState *L = g->learn;
+    const int64_t v1 = std::min(hi, L->big_n);   // renumbered: big vertices are ids [0, big_n)
+    if (lo >= v1) return GALP_OK;
+    const int gf = grid_cap(g, (v1 - lo + 3) / 4, 64);
+    if (g->off64)
+        k_features<int64_t><<<gf, 256, 0, g->stream>>>((const int64_t *)offrow, nbr, (const int64_t *)g->d_off,
+                                                       nullptr, lo, v1 - lo, L->d_fdeg, L->d_fmean, L->d_fmax);
+    else
+        k_features<int32_t><<<gf, 256, 0, g->stream>>>((const int32_t *)offrow, nbr, (const int32_t *)g->d_off,
+                                                       nullptr, lo, v1 - lo, L->d_fdeg, L->d_fmean, L->d_fmax);
+    return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "feature kernel launch failed");
 }
 
 int learn_reset_iteration(galp_graph *g) {
@@ -773,31 +800,40 @@ int learn_reset_iteration(galp_graph *g) {
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "reset kernels failed");
 }
 
+// Big vertices [v0, v0 + nv) of the big list, rows located by off.
 template <int H, typename E>
-static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl) {
-    if (!L->big_n) return;
-    k_dispatch<H, E><<<(int)((L->big_n + kDispChunk - 1) / kDispChunk), 256, 0, g->stream>>>(
-        (const E *)g->d_off, g->d_vlist, L->big_v0, L->big_n, L->d_params, L->d_fdeg, L->d_fmean, L->d_fmax,
+static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl, const E *off, int64_t v0,
+                            int64_t nv) {
+    if (!nv) return;
+    k_dispatch<H, E><<<(int)((nv + kDispChunk - 1) / kDispChunk), 256, 0, g->stream>>>(
+        off, g->d_vlist, v0, nv, !g->ooc, L->d_params, L->d_fdeg, L->d_fmean, L->d_fmax,
         L->d_dist[L->dcur], L->cand_thresh, g->nhubs, L->d_mt_of, L->d_mt_cand, L->d_hub_sel, L->d_cw_v,
         L->d_cw_e0, L->d_cw_len, L->d_cw_pslot, L->d_cw_base, dpl, L->d_hub_hd, L->d_plist_n,
         L->d_cand_bytes + g->iter % kMaxIters,
         g->d_stop);
 }
 
+// The dispatch and the candidate path over big vertices [v0, v0 + nv) (an
+// index range of the big list), rows located by off / nbr: the whole graph
+// in core, one batch's rebased offsets and buffer out of core.  The counters
+// and lists are reset even when the range is empty, since the class kernels
+// that follow read them.
 template <typename E>
-static int learned_front(galp_graph *g) {
+static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_t v0, int64_t nv, bool first) {
     LearnState *L = g->learn;
     int32_t *const *dpl = L->d_plist;
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
     PendingLaunch pl;
     int64_t *cb = L->d_cand_bytes + g->iter % kMaxIters;
-    k_reset_front<<<1, 64, 0, g->stream>>>(L->d_plist_n, cb);
+    k_reset_front<<<1, 64, 0, g->stream>>>(L->d_plist_n, cb, first ? 1 : 0);
+    L->front_n = nv;
+    if (!nv) return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "reset launch failed");
     prof_begin(g, "dispatch", 0.0, &pl);
     switch (L->H) {
-    case 8: launch_dispatch<8, E>(g, L, dpl); break;
-    case 16: launch_dispatch<16, E>(g, L, dpl); break;
-    default: launch_dispatch<32, E>(g, L, dpl); break;
+    case 8: launch_dispatch<8, E>(g, L, dpl, off, v0, nv); break;
+    case 16: launch_dispatch<16, E>(g, L, dpl, off, v0, nv); break;
+    default: launch_dispatch<32, E>(g, L, dpl, off, v0, nv); break;
     }
     prof_end(g, &pl);
     prof_begin_dev(g, "cand", cb, &pl);   // the dispatch counts the candidate path's bytes
@@ -817,10 +853,10 @@ static int learned_front(galp_graph *g) {
     cl.tile_mt = L->d_mt_tmt;
     cl.nfixed = L->nctiles;
     k_cand<E><<<g->ctx->prop.multiProcessorCount, 1024, 0, g->stream>>>(
-        (const E *)g->d_off, g->d_nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn,
-        L->distinct_out(g), dpl, L->d_plist_n, hot, g->d_counter, g->d_stop);
+        off, nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, L->distinct_out(g), dpl,
+        L->d_plist_n, hot, g->d_counter, g->d_stop);
     prof_end(g, &pl);
-    if (L->nmt) {
+    if (L->nmt && !g->ooc) {
         prof_begin(g, "cand_merge", 0.0, &pl);
         k_cand_merge<E><<<grid_cap(g, L->nmt, 4), 256, 0, g->stream>>>(
             (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
@@ -832,7 +868,17 @@ static int learned_front(galp_graph *g) {
 }
 
 int launch_learned_front(galp_graph *g) {
-    return g->off64 ? learned_front<int64_t>(g) : learned_front<int32_t>(g);
+    LearnState *L = g->learn;
+    return g->off64 ? learned_front<int64_t>(g, (const int64_t *)g->d_off, g->d_nbr, L->big_v0, L->big_n, true)
+                    : learned_front<int32_t>(g, (const int32_t *)g->d_off, g->d_nbr, L->big_v0, L->big_n, true);
+}
+
+int launch_learned_front_batch(galp_graph *g, const IterView &w, int64_t lo, int64_t hi) {
+    LearnState *L = g->learn;
+    const int64_t v1 = std::min(hi, L->big_n);   // renumbered: big vertices are ids [0, big_n)
+    const int64_t nv = lo < v1 ? v1 - lo : 0;
+    return g->off64 ? learned_front<int64_t>(g, (const int64_t *)w.off, w.nbr, lo, nv, lo == 0)
+                    : learned_front<int32_t>(g, (const int32_t *)w.off, w.nbr, lo, nv, lo == 0);
 }
 
 // After the iteration's LP kernels: log samples (if enabled), swap counts.

@@ -668,7 +668,7 @@ static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_
     // vertices are predicted high-distinct
     const bool learned = g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED;
     const int64_t *hd = learned ? g->learn->d_plist_n + kCtrHdAll : nullptr;
-    const int64_t hd_total = learned ? g->learn->big_n : 0;
+    const int64_t hd_total = learned ? g->learn->front_n : 0;   // big vertices of this dispatch
     if (g->d_vlist)
         k_group<G, HashGroupResolve, HashGroupSweep, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, hd, hd_total);

@@ -190,6 +190,10 @@ static int ooc_round(galp_graph *g) {
         IterView w = O->views[(size_t)b];
         w.off = O->d_offb[k];
         w.nbr = O->d_buf[k];
+        if (g->opts.method == GALP_METHOD_HASH && g->opts.dispatch == GALP_DISPATCH_LEARNED) {
+            const int rcf = launch_learned_front_batch(g, w, lo, hi);   // this batch's lists and candidates
+            if (rcf != GALP_OK) return rcf;
+        }
         const int rc = g->opts.method == GALP_METHOD_HASH ? launch_hash_iteration(g, w)
                                                             : launch_sort_iteration(g, w);
         if (rc != GALP_OK) return rc;
@@ -201,6 +205,30 @@ static int ooc_round(galp_graph *g) {
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("out-of-core launch: ") + hipGetErrorString(e));
+    return GALP_OK;
+}
+
+// Learned dispatch set-up: every batch is copied (synchronously, before any
+// iteration) and its big vertices' features computed from its rows.
+int ooc_features(galp_graph *g) {
+    OocState *O = g->ooc;
+    const int64_t nb = (int64_t)O->views.size();
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t e0 = O->e0[(size_t)b], ne = O->e1[(size_t)b] - e0;
+        const int64_t lo = O->bbs[(size_t)b], hi = O->bbs[(size_t)b + 1];
+        if (ne && hipMemcpyAsync(O->d_buf[0], O->h_nbr + e0, 4 * (size_t)ne, hipMemcpyHostToDevice, g->stream) !=
+                      hipSuccess)
+            return fail(g->ctx, GALP_EHIP, "out-of-core batch copy");
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 256) / 256, 1024));
+        if (g->off64)
+            k_rebase<int64_t><<<grid, 256, 0, g->stream>>>((const int64_t *)g->d_off, (int64_t *)O->d_offb[0], lo, hi);
+        else
+            k_rebase<int32_t><<<grid, 256, 0, g->stream>>>((const int32_t *)g->d_off, (int32_t *)O->d_offb[0], lo, hi);
+        const int rc = learn_features_view(g, O->d_offb[0], O->d_buf[0], lo, hi);
+        if (rc != GALP_OK) return rc;
+        // the next copy reuses the buffer
+        if (hipStreamSynchronize(g->stream) != hipSuccess) return fail(g->ctx, GALP_EHIP, "feature pass failed");
+    }
     return GALP_OK;
 }
 

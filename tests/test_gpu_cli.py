@@ -53,8 +53,10 @@ def test_hash_variants(tmp_path, variant):
     assert p.stderr.split("\t")[0] == "lfht" + variant
 
 
-def test_learned_dispatch(tmp_path):
-    p, labels = run(["-i", "3", "--dispatch", "learned", "1", "0", SAMPLE], tmp_path)
+@pytest.mark.parametrize("policy", ["0", "1", "2"])
+def test_learned_dispatch(tmp_path, policy):
+    # in-core and both out-of-core policies (batches of < 2^10 edges)
+    p, labels = run(["-i", "3", "-b", "10", "--dispatch", "learned", "1", policy, SAMPLE], tmp_path)
     assert p.returncode == 0, p.stdout + p.stderr
     assert labels == golden(3)
 

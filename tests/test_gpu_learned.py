@@ -136,12 +136,12 @@ def test_logged_distinct_counts_exact(ctx, reorder):
             prev, _ = oracle.lp_iterate(off, nbr, prev)
 
 
-def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, reorder=1):
-    say(f"[learned] {tag} n={len(off) - 1} m={len(nbr)} thresh={thresh} reorder={reorder}")
+def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, reorder=1, **opts):
+    say(f"[learned] {tag} n={len(off) - 1} m={len(nbr)} thresh={thresh} reorder={reorder} {opts}")
     n = len(off) - 1
     prev = np.arange(n, dtype=np.int32) if prev is None else prev
     routed = overflowed = 0
-    with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, reorder=reorder) as g:
+    with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, reorder=reorder, **opts) as g:
         g.set_mlp(hidden, params, thresh)
         g.set_labels(prev)
         for it in range(niter):
@@ -213,6 +213,62 @@ def test_learned_device_loop(ctx):
         labels, res = g.run(10)
     assert res["iterations"] == len(per)
     assert np.array_equal(labels, final)
+
+
+OOC = (_abi.POLICY_OOC_SYNC, _abi.POLICY_OOC_ASYNC)
+
+
+@pytest.mark.parametrize("policy", OOC)
+def test_learned_out_of_core_features(ctx, policy):
+    # features computed from the streamed batches equal the in-core ones
+    off, nbr = big_graph(6)
+    d, mean, mx = oracle.vertex_features(off, nbr)
+    big = np.diff(off) > 64
+    with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, policy=policy, buffer_pow=14) as g:
+        gd, gmean, gmx = g.features()
+    assert np.array_equal(gd[big], d[big])
+    assert np.array_equal(gmean[big], mean[big])
+    assert np.array_equal(gmx[big], mx[big])
+
+
+@pytest.mark.parametrize("policy", OOC)
+@pytest.mark.parametrize("net", ["all-cand", "no-cand", "random"])
+def test_learned_out_of_core(ctx, policy, net):
+    # per-batch dispatch: batches of < 2^14 edges (the 40k hub has a batch of
+    # its own); rows of one tile can take the candidate path, longer rows
+    # stay on the class and hub paths
+    off, nbr = big_graph(7)
+    if net == "all-cand":
+        routed, overflowed = run_learned(ctx, off, nbr, 5, f"ooc{policy}-all-cand", const_net(8, 0.0), 8, 4.0,
+                                         policy=policy, buffer_pow=14)
+        assert routed > 0 and overflowed > 0
+    elif net == "no-cand":
+        routed, overflowed = run_learned(ctx, off, nbr, 4, f"ooc{policy}-no-cand", const_net(8, 30.0), 8, 4.0,
+                                         policy=policy, buffer_pow=14)
+        assert routed == 0 and overflowed == 0
+    else:
+        run_learned(ctx, off, nbr, 6, f"ooc{policy}-rand", _abi.mlp_init(32, 5), 32, 3.0, policy=policy,
+                    buffer_pow=14)
+
+
+@pytest.mark.parametrize("policy", OOC)
+def test_learned_out_of_core_rmat_device_loop(ctx, policy):
+    off, nbr = _abi.gen_rmat(15, 16, seed=6)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    for bp in (12, 16):
+        say(f"[learned] ooc rmat15 policy={policy} b={bp}")
+        with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, policy=policy, buffer_pow=bp) as g:
+            g.set_mlp(8, const_net(8, 1.0), 4.0)
+            labels, res = g.run(10)
+        assert res["iterations"] == len(per)
+        assert np.array_equal(labels, final), bp
+
+
+def test_learned_out_of_core_needs_renumbering(ctx):
+    off, nbr = big_graph(8)
+    with pytest.raises(_abi.GalpError) as e:
+        _abi.Graph(ctx, off, nbr, dispatch=LEARNED, policy=_abi.POLICY_OOC_SYNC, reorder=0)
+    assert e.value.code == -4
 
 
 def test_learned_needs_a_network(ctx):
