@@ -836,7 +836,10 @@ static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_
     default: launch_dispatch<32, E>(g, L, dpl, off, v0, nv); break;
     }
     prof_end(g, &pl);
-    prof_begin_dev(g, "cand", cb, &pl);   // the dispatch counts the candidate path's bytes
+    // the dispatch counts the candidate path's bytes per iteration (out of
+    // core they add up over the batches, so no per-launch figure there)
+    if (g->ooc) prof_begin(g, "cand", 0.0, &pl);
+    else prof_begin_dev(g, "cand", cb, &pl);
     const int32_t hot = g->reorder ? (int32_t)std::min<int64_t>(kCandCache, g->n) : 0;
     CandLists cl{};
     for (int c = 0; c < kCandClasses; ++c) {
