@@ -7,8 +7,9 @@ namespace galp {
 
 // Degree <= G: one aligned group of G lanes per vertex; lane p holds the
 // label of the p-th neighbour (-1 past the degree).  Resolve::best<G>(L)
-// returns the group's max of (count << 8 | last), `last` being a lane index
-// (a CSR position in the row), so the winner is the label of lane `last`.
+// returns the group's max of (count << 8 | last), the same in every lane of
+// the group, `last` being a lane index (a CSR position in the row), so the
+// winner is the label of lane `last`.
 // One wave holds 64/G vertices per round; rounds are software-pipelined:
 // while round t is resolved, the labels of round t+1 and the neighbour ids
 // of round t+2 are loading, and the offsets of round t+3.
@@ -44,13 +45,13 @@ __device__ __forceinline__ uint32_t group_rounds(const E *__restrict__ off, cons
     };
     auto gat = [&](int32_t u, const St &s) -> int32_t { return prev[has(s) ? u : 0]; };
     uint32_t changed = 0;
+    // the winner lane (best's low byte) holds the new label and writes it
     auto resolve = [&](const St &s, int32_t Lraw, int32_t old) {
         const int32_t L = has(s) ? Lraw : -1;
         const uint32_t best = Resolve::template best<G>(L);
-        const int32_t win = __shfl(L, (int)(best & 0xffu), G);
-        if (p == 0 && s.ok && s.oe != s.ob) {
-            next[s.v] = win;
-            changed += (win != old);
+        if (p == (int)(best & 0xffu) && s.ok && s.oe != s.ob) {
+            next[s.v] = L;
+            changed += (L != old);
         }
     };
     int64_t base = (int64_t)blockIdx.x * per_block;   // uniform round index

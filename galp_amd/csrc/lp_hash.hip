@@ -42,54 +42,59 @@ const char *const kBucketName[NUM_BUCKETS] = {"g4",   "g8",   "g16",  "g32",  "g
 // Degree <= G: one aligned group of G lanes per vertex; lane p holds the
 // label of the p-th neighbour (or -1).  The group's max of (count << 8 |
 // last) is found in two stages:
-//   peel:  up to PEEL rounds; in each round every group takes the label of
-//          its first unresolved lane and one ballot gives that label's count
-//          (popcount) and last position (highest set bit), so the labels
-//          that dominate a neighbourhood are settled in one round each;
+//   peel:  up to PEEL rounds; in each round every group takes its smallest
+//          unresolved label (a min over lane moves) and one ballot gives that
+//          label's count (popcount) and last position (highest set bit), so
+//          the labels of a converged neighbourhood are settled in a round
+//          each, whatever the order;
 //   sweep: lanes still unresolved (a group with more than PEEL distinct
-//          labels) compare against the G lanes of their group.
+//          labels) count the lanes of their group holding their label; the
+//          max over those lanes of (count << 8 | lane) is the max over their
+//          labels of (count << 8 | last), since a label's highest lane is its
+//          last position.
 // A peeled label takes all of its lanes with it, so the sweep's counts
-// among unresolved lanes are full counts.
+// among unresolved lanes are full counts.  The result is the same for every
+// lane of the group.
 template <int G, int PEEL>
 __device__ __forceinline__ uint32_t group_best(int32_t L) {
     const int lane = lane_id();
     const int gbase = lane & ~(G - 1);
-    const unsigned long long gbits = (G == kWave) ? ~0ull : (((1ull << G) - 1ull) << gbase);
     bool active = L >= 0;
     uint32_t best = 0;
 #pragma unroll
     for (int r = 0; r < PEEL; ++r) {
         const unsigned long long ball = __ballot(active);
-        if (ball == 0) break;
+        if (ball == 0) break;   // wave-uniform
         int32_t Lk;
-        if constexpr (G == kWave) {
-            Lk = __builtin_amdgcn_readlane(L, __ffsll((long long)ball) - 1);
-        } else {
-            const unsigned long long seg = ball & gbits;
-            Lk = __shfl(L, seg ? __ffsll((long long)seg) - 1 : lane);
-        }
+        if constexpr (G == kWave) Lk = __builtin_amdgcn_readlane(L, __ffsll((long long)ball) - 1);
+        else Lk = group_min<G>(active ? L : INT32_MAX);
         const bool eq = active && (L == Lk);
-        const unsigned long long m = __ballot(eq) & gbits;
-        if (m) {
-            const uint32_t cnt = (uint32_t)__popcll(m);
-            const uint32_t last = (uint32_t)(63 - __clzll((long long)m) - gbase);
-            best = max(best, (cnt << 8) | last);
+        const unsigned long long b = __ballot(eq);
+        uint32_t cnt, last;
+        bool any;
+        if constexpr (G == kWave) {
+            any = b != 0ull;
+            cnt = (uint32_t)__popcll(b);
+            last = (uint32_t)(63 - __clzll((long long)(b | 1ull)));
+        } else {
+            const uint32_t mg = (uint32_t)(b >> gbase) & (uint32_t)((1ull << G) - 1ull);
+            any = mg != 0u;
+            cnt = (uint32_t)__popc(mg);
+            last = (uint32_t)(31 - __clz((int)(mg | 1u)));
         }
+        best = any ? max(best, (cnt << 8) | last) : best;
         active = active && !eq;
     }
     if (__ballot(active)) {
         const int32_t La = active ? L : -1;
-        uint32_t cnt = 0, last = 0;
+        uint32_t cnt = 0;
+        if constexpr (G == kWave) {
 #pragma unroll
-        for (int j = 0; j < G; ++j) {
-            int32_t Lj;
-            if constexpr (G == kWave) Lj = __builtin_amdgcn_readlane(La, j);
-            else Lj = __shfl(La, j, G);
-            const bool eq = active && (Lj == L);
-            cnt += eq;
-            last = eq ? (uint32_t)j : last;
+            for (int j = 0; j < kWave; ++j) cnt += (uint32_t)(__builtin_amdgcn_readlane(La, j) == La);
+        } else {
+            cnt = group_count_eq<G>(La);
         }
-        best = max(best, group_max<G>(active ? ((cnt << 8) | last) : 0u));
+        best = max(best, group_max<G>(active ? ((cnt << 8) | (uint32_t)(lane & (G - 1))) : 0u));
     }
     return best;
 }

@@ -67,6 +67,20 @@ def test_batched_equals_independent_runs(ctx, method, reorder):
         assert np.array_equal(labels, labels2)
 
 
+def test_batched_autotuned_captured(ctx):
+    # the batched configuration of the benchmark: autotuned class kernels and
+    # the captured loop; per-graph labels stay exact
+    parts, vbase, off, nbr = make_batch(4, 20)
+    say(f"[batch] autotuned captured graphs=20 n={len(off) - 1} m={len(nbr)}")
+    with _abi.Graph(ctx, off, nbr, reorder=0, graph_capture=1) as g:
+        g.set_batch(vbase)
+        chosen = g.autotune(reps=2)
+        assert len(chosen) == 12 and all(0 <= c <= 2 for c in chosen)
+        for _ in range(2):
+            labels, res = g.run(10)
+            check_batch(labels, g.batch_iterations(), parts, vbase, 10)
+
+
 def test_batch_rejects_crossing_edges(ctx):
     off = np.array([0, 1, 2, 3, 4], np.int64)
     nbr = np.array([1, 0, 3, 0], np.int32)   # vertex 3 -> 0 crosses graphs {0,1} | {2,3}
