@@ -111,7 +111,7 @@ constexpr int kHdDiv = 8;
 // kCandTile edges keeping up to kCandK labels in registers.
 constexpr int kCandTile = 1024;
 constexpr int kCandK = 16;     // candidates per tile (lanes of the candidate table, <= 64)
-constexpr int kCandCache = 32768;   // hottest labels held in LDS per workgroup
+constexpr int kCandCache = 38912;   // hottest labels held in LDS per workgroup (152 KB)
 
 // State of the learned dispatcher (allocated when dispatch is learned or
 // training samples are logged).  "Big" vertices are those of the classes
