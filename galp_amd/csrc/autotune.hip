@@ -1,16 +1,17 @@
 // autotune.hip -- picks each degree class's kernel configuration by timing.
 //
 // No reference counterpart (the reference fixes its block sizes at compile
-// time, lfht.cuh:127-165).  For every class of the hash method with
-// vertices, hubs included, each of its configurations (launch_class in
-// lp_hash.hip: workgroup size x slots per lane and blocks per CU; for hubs
-// the tile and merge workgroup sizes) is launched `reps`
-// times on the graph's current labels and timed with HIP events; the
-// fastest is kept.  The choice never changes labels (every configuration
-// computes the same rule), which is how it is tested.  Choices can be
-// cached in a text file, one line per class:
-//   <arch> <class> <log2 vertices> <log2 edges> <option>
-// and are reused for a graph whose class has the same arch and sizes.
+// time, lfht.cuh:127-165).  For every degree class with vertices, hubs
+// included, each of its configurations is launched `reps` times on the
+// graph's current labels and timed with HIP events; the fastest is kept.
+// The configurations are the method's: launch_class in lp_hash.hip
+// (workgroup size x slots per lane and blocks per CU; for hubs the tile and
+// merge workgroup sizes) and sort_class in lp_sort.hip (workgroup size x
+// keys per thread, radix or bitonic, grids).  The choice never changes
+// labels (every configuration computes the same rule), which is how it is
+// tested.  Choices can be cached in a text file, one line per class:
+//   <arch> <method> <class> <log2 vertices> <log2 edges> <option>
+// and are reused for a graph whose class has the same arch, method and sizes.
 
 #include <cmath>
 #include <cstdio>
@@ -29,7 +30,7 @@ static int ilog2(int64_t x) { return x > 0 ? 63 - __builtin_clzll((unsigned long
 
 struct CacheLine {
     std::string arch;
-    int cls, lv, le, opt;
+    int method, cls, lv, le, opt;
 };
 
 static std::vector<CacheLine> read_cache(const char *path) {
@@ -40,8 +41,8 @@ static std::vector<CacheLine> read_cache(const char *path) {
     while (std::getline(f, line)) {
         std::istringstream is(line);
         CacheLine c;
-        if (is >> c.arch >> c.cls >> c.lv >> c.le >> c.opt && c.cls >= 0 && c.cls < NUM_BUCKETS && c.opt >= 0 &&
-            c.opt < kTuneOptions)
+        if (is >> c.arch >> c.method >> c.cls >> c.lv >> c.le >> c.opt && c.cls >= 0 && c.cls < NUM_BUCKETS &&
+            c.opt >= 0 && c.opt < kTuneOptions)
             out.push_back(c);
     }
     return out;
@@ -64,8 +65,9 @@ GALP_API int galp_graph_set_tuning(galp_graph *g, const int32_t *options) {
 
 GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_path, int32_t *chosen) {
     if (!g || reps < 1) return GALP_EINVAL;
-    if (g->opts.method != GALP_METHOD_HASH || g->opts.variant != 1 || g->ooc)
-        return fail(g->ctx, GALP_EUNSUPPORTED, "the autotuner tunes method 1, -l 1, in-core");
+    const bool hash = g->opts.method == GALP_METHOD_HASH;
+    if ((hash && g->opts.variant != 1) || g->opts.policy != GALP_POLICY_INCORE || g->ooc)
+        return fail(g->ctx, GALP_EUNSUPPORTED, "the autotuner tunes policy 0 (in-core), and method 1 with -l 1");
     {   // tune on labels[i] = i, the first iteration's (and heaviest) workload
         const int rc0 = galp_lp_init(g);
         if (rc0 != GALP_OK) return rc0;
@@ -87,7 +89,7 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
         const int lv = ilog2(r.nv), le = ilog2(r.edges);
         bool hit = false;
         for (const CacheLine &c : cache)
-            if (c.arch == arch && c.cls == b && c.lv == lv && c.le == le) {
+            if (c.arch == arch && c.method == g->opts.method && c.cls == b && c.lv == lv && c.le == le) {
                 g->tune[b] = (int8_t)c.opt;
                 hit = true;
             }
@@ -96,10 +98,11 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
         int bo = 0;
         for (int o = 0; o < kTuneOptions && rc == GALP_OK; ++o) {
             g->tune[b] = (int8_t)o;
-            rc = launch_hash_class(g, b);   // warm-up launch
+            rc = hash ? launch_hash_class(g, b) : launch_sort_class(g, b);   // warm-up launch
             if (rc != GALP_OK) break;
             (void)hipEventRecord(e0, g->stream);
-            for (int k = 0; k < reps && rc == GALP_OK; ++k) rc = launch_hash_class(g, b);
+            for (int k = 0; k < reps && rc == GALP_OK; ++k)
+                rc = hash ? launch_hash_class(g, b) : launch_sort_class(g, b);
             (void)hipEventRecord(e1, g->stream);
             if (hipEventSynchronize(e1) != hipSuccess) rc = fail(g->ctx, GALP_EHIP, "autotune timing");
             float ms = 0.f;
@@ -110,7 +113,7 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
             }
         }
         g->tune[b] = (int8_t)bo;
-        fresh.push_back(CacheLine{arch, b, lv, le, bo});
+        fresh.push_back(CacheLine{arch, g->opts.method, b, lv, le, bo});
     }
     g->prof.on = prof;
     g->opts.dispatch = dispatch;
@@ -123,7 +126,8 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
     if (rc != GALP_OK) return rc;
     if (cache_path && !fresh.empty()) {
         std::ofstream f(cache_path, std::ios::app);
-        for (const CacheLine &c : fresh) f << c.arch << ' ' << c.cls << ' ' << c.lv << ' ' << c.le << ' ' << c.opt << '\n';
+        for (const CacheLine &c : fresh)
+            f << c.arch << ' ' << c.method << ' ' << c.cls << ' ' << c.lv << ' ' << c.le << ' ' << c.opt << '\n';
     }
     if (chosen)
         for (int b = 0; b < NUM_BUCKETS; ++b) chosen[b] = g->tune[b];

@@ -292,6 +292,7 @@ int launch_iter_end(galp_graph *g);
 int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg);
 void sort_free(galp_graph *g);
 int launch_sort_iteration(galp_graph *g, const IterView &w);
+int launch_sort_class(galp_graph *g, int b);   // one class of the whole graph (the autotuner's unit)
 
 // Out-of-core policies (ooc.hip).
 int ooc_setup(galp_graph *g, const galp_csr_view *c, const std::vector<int64_t> &offi,

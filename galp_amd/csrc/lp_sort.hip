@@ -846,7 +846,8 @@ static double class_bytes(const BucketRange &r) {
 }
 
 template <int G, typename E>
-static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next) {
+static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
+                       int blocks_per_cu = 8) {
     const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
@@ -854,10 +855,12 @@ static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t 
     const int64_t per_block = 256 / G;
     const int64_t blocks = (r.nv + per_block - 1) / per_block;
     if (g->d_vlist)
-        k_group<G, SortGroupResolve, SortGroupResolve, true, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+        k_group<G, SortGroupResolve, SortGroupResolve, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0,
+                                                                  g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, nullptr, 0);
     else
-        k_group<G, SortGroupResolve, SortGroupResolve, false, E><<<grid_for(g, blocks, 8), 256, 0, g->stream>>>(
+        k_group<G, SortGroupResolve, SortGroupResolve, false, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0,
+                                                                   g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop, nullptr, 0);
     prof_end(g, &pl);
 }
@@ -884,10 +887,10 @@ static void sort_lds(galp_graph *g, const IterView &w, int bid, const int32_t *p
 // pass cover just those tiles; the scan starts at the hubs' first compact
 // slot, so the output lands at the same places as in a whole-graph pass.
 template <typename E>
-static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next) {
+static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next, int tiles_per_cu = 8) {
     SortState *S = g->sort;
     const int64_t t0 = w.t0, t1 = w.t1, nt = t1 - t0, nh = w.h1 - w.h0;
-    const int gt = grid_for(g, nt, 8);
+    const int gt = grid_for(g, nt, tiles_per_cu);
     const int64_t nscan = nt * 256;
     const int nbs = (int)((nscan + kScanChunk - 1) / kScanChunk);
     uint32_t *hist = S->d_hist + t0 * 256;
@@ -926,31 +929,88 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
     return GALP_OK;
 }
 
+// The sort method's configuration of each degree class (g->tune[b]): option
+// 0 is the default, 1 and 2 the alternatives the autotuner times --
+// workgroup size x keys per thread, the radix sort or the bitonic network,
+// blocks per CU; for lane groups the grid; for hubs the tiles per CU of the
+// histogram and scatter grids.  Every option computes the same labels.
+template <typename E>
+static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
+    const int c = g->tune[b];
+    switch (b) {
+    case B_HUB:
+        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 4 : c == 2 ? 16 : 8);
+        break;
+    case B_B4K:
+        if (c == 1) sort_lds<1024, 4, E, true>(g, w, b, prev, next, 1);
+        else if (c == 2) sort_lds<256, 16, E, true>(g, w, b, prev, next, 4);
+        else sort_lds<512, 8, E, true>(g, w, b, prev, next, 2);
+        break;
+    case B_B2K:
+        if (c == 1) sort_lds<512, 4, E, true>(g, w, b, prev, next, 2);
+        else if (c == 2) sort_lds<128, 16, E, true>(g, w, b, prev, next, 8);
+        else sort_lds<256, 8, E, true>(g, w, b, prev, next, 4);
+        break;
+    case B_B1K:
+        if (c == 1) sort_lds<256, 4, E, false>(g, w, b, prev, next, 4);
+        else if (c == 2) sort_lds<128, 8, E, true>(g, w, b, prev, next, 8);
+        else sort_lds<128, 8, E, false>(g, w, b, prev, next, 8);
+        break;
+    case B_W512:
+        if (c == 1) sort_lds<128, 4, E, false>(g, w, b, prev, next, 8);
+        else if (c == 2) sort_lds<64, 8, E, true>(g, w, b, prev, next, 16);
+        else sort_lds<64, 8, E, false>(g, w, b, prev, next, 16);
+        break;
+    case B_W256:
+        if (c == 1) sort_lds<128, 2, E, false>(g, w, b, prev, next, 12);
+        else if (c == 2) sort_lds<64, 4, E, true>(g, w, b, prev, next, 24);
+        else sort_lds<64, 4, E, false>(g, w, b, prev, next, 24);
+        break;
+    case B_W128:
+        if (c == 1) sort_lds<64, 2, E, false>(g, w, b, prev, next, 16);
+        else if (c == 2) sort_lds<64, 2, E, true>(g, w, b, prev, next, 32);
+        else sort_lds<64, 2, E, false>(g, w, b, prev, next, 32);
+        break;
+    default: {
+        const int bpc = c == 1 ? 16 : c == 2 ? 4 : 8;
+        if (b == B_G64) sort_group<64, E>(g, w, b, prev, next, bpc);
+        else if (b == B_G32) sort_group<32, E>(g, w, b, prev, next, bpc);
+        else if (b == B_G16) sort_group<16, E>(g, w, b, prev, next, bpc);
+        else if (b == B_G8) sort_group<8, E>(g, w, b, prev, next, bpc);
+        else sort_group<4, E>(g, w, b, prev, next, bpc);
+    }
+    }
+    return GALP_OK;
+}
+
+int launch_sort_class(galp_graph *g, int b) {
+    const IterView w = full_view(g);
+    const int32_t *prev = g->d_lab[g->cur];
+    int32_t *next = g->d_lab[g->cur ^ 1];
+    const int rc = g->off64 ? sort_class<int64_t>(g, w, b, prev, next) : sort_class<int32_t>(g, w, b, prev, next);
+    if (rc != GALP_OK) return rc;
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("class launch: ") + hipGetErrorString(e));
+}
+
 template <typename E>
 static int sort_iteration(galp_graph *g, const IterView &w) {
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
-    if (w.h1 > w.h0) {
-        const int rc = sort_hubs<E>(g, w, prev, next);
-        if (rc != GALP_OK) return rc;
-    }
     if (g->opts.policy == GALP_POLICY_LI) {
         // "load-imbalanced" ablation: every class <= 4096 through the
         // 512-thread block-per-vertex kernel, as the reference's li path
         // runs one block per vertex whatever its degree.
+        if (w.h1 > w.h0) {
+            const int rc = sort_hubs<E>(g, w, prev, next);
+            if (rc != GALP_OK) return rc;
+        }
         for (int b = B_B4K; b >= B_G4; --b) sort_lds<512, 8, E, true>(g, w, b, prev, next, 2);
     } else {
-        sort_lds<512, 8, E, true>(g, w, B_B4K, prev, next, 2);
-        sort_lds<256, 8, E, true>(g, w, B_B2K, prev, next, 4);
-        sort_lds<128, 8, E, false>(g, w, B_B1K, prev, next, 8);
-        sort_lds<64, 8, E, false>(g, w, B_W512, prev, next, 16);
-        sort_lds<64, 4, E, false>(g, w, B_W256, prev, next, 24);
-        sort_lds<64, 2, E, false>(g, w, B_W128, prev, next, 32);
-        sort_group<64, E>(g, w, B_G64, prev, next);
-        sort_group<32, E>(g, w, B_G32, prev, next);
-        sort_group<16, E>(g, w, B_G16, prev, next);
-        sort_group<8, E>(g, w, B_G8, prev, next);
-        sort_group<4, E>(g, w, B_G4, prev, next);
+        for (int b = B_HUB; b >= B_G4; --b) {   // largest work first
+            const int rc = sort_class<E>(g, w, b, prev, next);
+            if (rc != GALP_OK) return rc;
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("sort kernel launch: ") + hipGetErrorString(e));

@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("option", [0, 1, 2])
 @pytest.mark.parametrize("reorder", [1, 0])
-def test_every_option_same_labels(ctx, option, reorder):
+@pytest.mark.parametrize("method", [_abi.METHOD_HASH, _abi.METHOD_SORT])
+def test_every_option_same_labels(ctx, option, reorder, method):
     off, nbr = hub_graph(5)
     # extra rows of 600..4000 edges so B1K..B4K have vertices
     rng = np.random.default_rng(option)
@@ -27,8 +28,8 @@ def test_every_option_same_labels(ctx, option, reorder):
     off = np.zeros(n + 1, np.int64)
     off[1:] = np.cumsum([len(r) for r in rows])
     nbr = np.array([u for r in rows for u in r], np.int32)
-    print(f"[tune] option {option} reorder {reorder}", flush=True)
-    with _abi.Graph(ctx, off, nbr, reorder=reorder) as g:
+    print(f"[tune] method {method} option {option} reorder {reorder}", flush=True)
+    with _abi.Graph(ctx, off, nbr, reorder=reorder, method=method) as g:
         g.set_tuning([option] * 12)
         g.init()
         prev = np.arange(n, dtype=np.int32)
@@ -40,18 +41,20 @@ def test_every_option_same_labels(ctx, option, reorder):
             prev = ref
 
 
-def test_autotune_and_cache(ctx, tmp_path):
+@pytest.mark.parametrize("method", [_abi.METHOD_HASH, _abi.METHOD_SORT])
+def test_autotune_and_cache(ctx, tmp_path, method):
     off, nbr = _abi.gen_rmat(16, 16, seed=21)
     final, per, counts = oracle.lp_run(off, nbr, niter=10)
     cache = str(tmp_path / "tune.txt")
-    with _abi.Graph(ctx, off, nbr) as g:
+    print(f"[tune] autotune method {method}", flush=True)
+    with _abi.Graph(ctx, off, nbr, method=method) as g:
         chosen = g.autotune(reps=2, cache_path=cache)
         assert all(0 <= c <= 2 for c in chosen)
         labels, res = g.run(10)
         assert np.array_equal(labels, final)
     lines = open(cache).read().split("\n")
     assert len([x for x in lines if x.strip()]) >= 5
-    with _abi.Graph(ctx, off, nbr) as g2:
+    with _abi.Graph(ctx, off, nbr, method=method) as g2:
         chosen2 = g2.autotune(reps=2, cache_path=cache)   # every class hits the cache
         assert np.array_equal(chosen, chosen2)
         labels, _ = g2.run(10)
@@ -67,3 +70,25 @@ def test_autotune_learned_graph(ctx):
         g.autotune(reps=1)
         labels, _ = g.run(10)
         assert np.array_equal(labels, final)
+
+
+def test_autotune_cache_keeps_methods_apart(ctx, tmp_path):
+    # a cache written by one method is not reused by the other
+    off, nbr = _abi.gen_rmat(14, 16, seed=23)
+    cache = str(tmp_path / "tune.txt")
+    with _abi.Graph(ctx, off, nbr, method=_abi.METHOD_HASH) as g:
+        g.autotune(reps=1, cache_path=cache)
+    n_hash = len([x for x in open(cache).read().split("\n") if x.strip()])
+    with _abi.Graph(ctx, off, nbr, method=_abi.METHOD_SORT) as g:
+        g.autotune(reps=1, cache_path=cache)
+    lines = [x.split() for x in open(cache).read().split("\n") if x.strip()]
+    assert len(lines) == 2 * n_hash
+    assert {ln[1] for ln in lines} == {str(_abi.METHOD_HASH), str(_abi.METHOD_SORT)}
+
+
+def test_autotune_refuses_other_policies(ctx):
+    off, nbr = _abi.gen_rmat(12, 16, seed=24)
+    with _abi.Graph(ctx, off, nbr, method=_abi.METHOD_SORT, policy=_abi.POLICY_LI) as g:
+        with pytest.raises(_abi.GalpError) as e:
+            g.autotune(reps=1)
+        assert e.value.code == -4
