@@ -15,7 +15,13 @@
 // plain class list in the same iteration (hub tiles: the hub to the plain
 // hub path), so a misprediction costs time, never labels.  Every path
 // records the number of distinct labels it saw, the network's fourth input
-// for the next iteration.
+// for the next iteration.  The prediction also picks kernel variants: rows
+// predicted high-distinct (more than degree / kHdDiv labels) go to lists
+// whose table kernels insert lane by lane, and the lane-group kernels switch
+// to their sweep-only resolver when most big vertices are high-distinct.
+// Out of core, each batch runs its own dispatch and candidate path over its
+// rebased rows (launch_learned_front_batch); rows of several candidate tiles
+// stay on the class and hub paths there.
 // No reference counterpart: the reference picks a kernel by -l only
 // (lfht.cuh:127-165).
 
