@@ -8,7 +8,9 @@
 - labels bit-exact against the oracle after every iteration whatever the
   network predicts: everything to the candidate path (thresholds far above
   any count, so multi-label rows overflow and take the spill path), nothing,
-  and seeded random networks in between.
+  and seeded random networks in between, in core and out of core;
+- what a network trained on the GPU gains over plain dispatch on RMAT-22
+  (same labels, less device time).
 """
 import sys
 
@@ -269,6 +271,42 @@ def test_learned_out_of_core_needs_renumbering(ctx):
     with pytest.raises(_abi.GalpError) as e:
         _abi.Graph(ctx, off, nbr, dispatch=LEARNED, policy=_abi.POLICY_OOC_SYNC, reorder=0)
     assert e.value.code == -4
+
+
+def test_trained_network_gains_over_plain_dispatch(ctx):
+    """What the trained network gains: it is trained on the GPU from a logged
+    plain run of RMAT-22 (the benchmark graph; at smaller scales launch
+    overheads hide the difference), predicts the distinct-label counts far
+    better than a constant, and the learned dispatch it drives gives the
+    oracle's labels in less device time than plain dispatch on the same
+    graph."""
+    off, nbr = _abi.gen_rmat(22, 16, seed=22)
+    big = int((np.diff(off) > 64).sum())
+    with _abi.Graph(ctx, off, nbr) as g:
+        g.log_enable(11 * big)
+        g.run(10, want_labels=False)
+        x, y = g.log_read()
+    p, loss = _abi.mlp_train(ctx, 32, _abi.mlp_init(32, 1), x, y, epochs=8, batch=512, lr=3e-3, seed=2)
+    pred = _abi.mlp_predict(ctx, 32, p, x)
+    mse, mse_const = float(np.mean((pred - y) ** 2)), float(np.var(y))
+    say(f"[learned] RMAT-22: {len(x)} samples, network mse {mse:.4f} vs constant {mse_const:.4f}")
+    assert mse < 0.1 * mse_const
+    final, _, _ = oracle.lp_run(off, nbr, niter=10)
+    times = {}
+    for name, disp in (("plain", _abi.DISPATCH_PLAIN), ("learned", LEARNED)):
+        with _abi.Graph(ctx, off, nbr, dispatch=disp) as g:
+            if disp == LEARNED:
+                g.set_mlp(32, p, 14.0)
+            g.run(10, want_labels=False)   # warm-up
+            ts = []
+            for _ in range(5):
+                labels, res = g.run(10)
+                assert np.array_equal(labels, final), name
+                ts.append(res["device_ms"])
+            times[name] = float(np.median(ts))
+    gain = times["plain"] / times["learned"]
+    say(f"[learned] RMAT-22: plain {times['plain']:.3f} ms, learned {times['learned']:.3f} ms, gain {gain:.3f}x")
+    assert gain > 1.05
 
 
 def test_learned_needs_a_network(ctx):
