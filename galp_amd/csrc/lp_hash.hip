@@ -764,14 +764,14 @@ static void launch_class(galp_graph *g, const IterView &w, int b, const int32_t 
         else launch_lds<512, 8, 13, E>(g, w, b, prev, next, 2);
         break;
     case B_B2K:
-        if (c == 1) launch_lds<512, 4, 12, E>(g, w, b, prev, next, 3);
+        if (c == 1) launch_lds<256, 8, 12, E>(g, w, b, prev, next, 4);
         else if (c == 2) launch_lds<128, 16, 12, E>(g, w, b, prev, next, 8);
-        else launch_lds<256, 8, 12, E>(g, w, b, prev, next, 4);
+        else launch_lds<512, 4, 12, E>(g, w, b, prev, next, 3);
         break;
     case B_B1K:
-        if (c == 1) launch_lds<256, 4, 11, E>(g, w, b, prev, next, 6);
+        if (c == 1) launch_lds<128, 8, 11, E>(g, w, b, prev, next, 8);
         else if (c == 2) launch_lds<64, 16, 11, E>(g, w, b, prev, next, 16);
-        else launch_lds<128, 8, 11, E>(g, w, b, prev, next, 8);
+        else launch_lds<256, 4, 11, E>(g, w, b, prev, next, 6);
         break;
     case B_W512:
         if (c == 1) launch_lds<128, 4, 10, E>(g, w, b, prev, next, 12);
@@ -789,7 +789,11 @@ static void launch_class(galp_graph *g, const IterView &w, int b, const int32_t 
         else launch_lds<64, 2, 8, E>(g, w, b, prev, next, 32);
         break;
     default: {   // lane groups: blocks per CU of the persistent grid
-        const int bpc = c == 1 ? 16 : c == 2 ? 4 : 8;
+        // the defaults are the autotuner's usual picks on RMAT, LFR and
+        // batched planted partitions: 4 per CU for G4, 16 for G64, else 8
+        const int def = b == B_G4 ? 4 : b == B_G64 ? 16 : 8;
+        const int alt1 = b == B_G64 ? 8 : 16, alt2 = b == B_G4 ? 8 : 4;
+        const int bpc = c == 1 ? alt1 : c == 2 ? alt2 : def;
         if (b == B_G64) launch_group<64, E>(g, w, b, prev, next, bpc);
         else if (b == B_G32) launch_group<32, E>(g, w, b, prev, next, bpc);
         else if (b == B_G16) launch_group<16, E>(g, w, b, prev, next, bpc);
