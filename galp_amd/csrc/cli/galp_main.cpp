@@ -41,7 +41,7 @@ static void usage(const char *prog) {
                  "Options:\n"
                  " -b n: Set buffer sizes to 2^n. The default is 24.\n"
                  " -i n: Specify the number of iterations. The default is 10.\n"
-                 " -g n: Use n GPUs. The default is 1 (the only value this build runs).\n"
+                 " -g n: Use n GPUs. The default is 1 (the only value this program runs).\n"
                  " -l n: Hash variant 0..3 (naive, fused, LDS+spill, load-balanced). Default 1.\n"
                  " --dispatch plain|learned  kernel choice by degree thresholds or the MLP\n"
                  " --no-reorder              keep the input vertex numbering internally\n"

@@ -112,7 +112,7 @@ __global__ void k_reset_front(int64_t *ctr, int64_t *cand_bytes, int first) {
 // one global atomicAdd per list, then hands out positions from LDS cursors.
 // (Same-address global atomics serialise at the L2; one per wave and list
 // cost ~100 us per iteration on RMAT-22, one per block and list does not.)
-constexpr int kDispChunk = 256;   // one vertex per thread: the MLP is LDS-latency bound
+constexpr int kDispChunk = 256;   // one vertex per thread
 // codes: [0, NUM_BUCKETS) peeled class list, [NUM_BUCKETS, 2 NUM_BUCKETS)
 // high-distinct class list, then candidate tile, multi-tile candidate, hub
 enum : uint8_t { kCodeCand = 2 * NUM_BUCKETS, kCodeCandMt, kCodeHub };
@@ -128,8 +128,9 @@ constexpr int kSlots = kSlotHdAll + 1;
 template <int H, typename E>
 __global__ void __launch_bounds__(256)
 k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t big_v0, int64_t big_n,
-           bool allow_mt, const float *__restrict__ params, const float *__restrict__ fdeg, const float *__restrict__ fmean,
-           const float *__restrict__ fmax, const int32_t *__restrict__ dprev, float cand_thresh, int64_t nhubs,
+           bool allow_mt, const float *__restrict__ params, const float *__restrict__ fdeg,
+           const float *__restrict__ fmean, const float *__restrict__ fmax, const int32_t *__restrict__ dprev,
+           float cand_thresh, int64_t nhubs,
            const int32_t *__restrict__ mt_of, int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel,
            int32_t *__restrict__ cw_v, int64_t *__restrict__ cw_e0, int32_t *__restrict__ cw_len,
            int32_t *__restrict__ cw_pslot, const int64_t *__restrict__ cw_base, int32_t *const *__restrict__ plist,
@@ -272,7 +273,7 @@ struct CandPolicy {
     __device__ int64_t edge0(const Item &it) const { return it.e0; }
 
     // lab holds prev[u] for cold neighbours; hot ones are re-read from LDS
-    // (the driver's gather still issued a load, to prev[0] for them, which
+    // (the pipeline's gather still issued a load, to prev[0] for them, which
     // all such lanes share, so it costs one line instead of one per lane)
     // Candidates live in lanes: lane j < ncand holds candidate j's label,
     // count and last position.  Each 64-edge chunk is peeled: one round per

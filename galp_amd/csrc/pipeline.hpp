@@ -13,7 +13,7 @@ namespace galp {
 //   Item item(int64_t i)               -- uniform per team (scalar loads)
 //   int64_t edge0(const Item &)        -- absolute index of the item's edge 0
 //   void process(const Item &, const int32_t (&lab)[EPT])
-// The driver keeps: labels of item t (being processed), neighbour ids of
+// The pipeline keeps: labels of item t (being processed), neighbour ids of
 // item t+1, descriptor of item t+2; a period-2 unroll ping-pongs the label
 // registers so no in-flight load is ever copied.
 template <int T, int EPT, class Policy>

@@ -56,7 +56,7 @@ GALP_API int galp_save_bin(const char *path, const galp_host_csr *g);
  * uniformly from the other communities; both directions are stored,
  * duplicates removed, rows sorted ascending.  Deterministic in seed
  * (SplitMix64 streams, Steele/Lea/Flood 2014).  No reference counterpart:
- * the reference ships no generator; this builds the synthetic inputs. */
+ * the reference ships no generator; this makes the synthetic inputs. */
 GALP_API int galp_gen_planted_partition(int64_t n, int64_t community_size,
                                         int32_t k_intra, int32_t k_inter,
                                         uint64_t seed, galp_host_csr **out);

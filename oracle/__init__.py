@@ -1,7 +1,7 @@
 """CPU oracle for galp's label-propagation update (ctypes wrapper over lp_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+cpu_baseline leg of bench.py, always as the parity reference or the timed CPU baseline.
 The product (libgalp, the galp CLI, galp_amd) never imports it.
 
 Each function cites the reference code it restates; see lp_oracle.c.

@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libgalp, the galp CLI,
  * the Python bindings) links, loads or calls this file.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only
- * as the checker / the timed CPU baseline, never as the thing being measured.
+ * as the parity reference / the timed CPU baseline, never as the thing being measured.
  *
  * What it restates (reference paths relative to the reference checkout):
  *   - loop structure of the serial CPU LP: label_propagation/include/
@@ -85,7 +85,7 @@ int64_t oracle_lp_iterate(int64_t n, const int64_t *off, const int32_t *nbr,
     return changed;
 }
 
-/* The driver of label_propagator.h:37-82 / lfht.cuh:73-112 made synchronous:
+/* The run loop of label_propagator.h:37-82 / lfht.cuh:73-112 made synchronous:
  * labels[i] = i, then up to `niter` iterations, stopping after an iteration
  * whose change count is <= n * 1e-5 (lfht.cuh:97, dpp.cuh:88).
  * per_iter (niter*n, may be NULL) receives the labels after each executed
