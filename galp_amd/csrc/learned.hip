@@ -449,95 +449,70 @@ k_cand(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t
 }
 
 // Multi-tile vertices on the candidate path: merge the tiles' partial
-// candidates (one block per vertex, LDS table).  An overflowed tile, or more
-// distinct labels than the table holds, hands the vertex to the plain path
-// of its class (hubs: the plain hub path), which runs after this kernel.
+// candidates, one wave per vertex.  The merged labels live in lanes (lane j
+// holds label j with its summed count and largest position); the tiles'
+// entries are folded in one at a time, a ballot finding the lane of an equal
+// label.  A tile that overflowed, or more than 64 distinct labels in all,
+// hands the vertex to the plain path of its class (hubs: the plain hub
+// path), which runs after this kernel.  Counts add and positions take the
+// max, so the merged (count, last) of every label is exact.
 template <typename E>
 __global__ void __launch_bounds__(256)
 k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int64_t nmt, int64_t nhubs,
              const int32_t *__restrict__ mt_cand, int32_t *__restrict__ hub_sel,
              const int32_t *__restrict__ mt_ctile0, const int32_t *__restrict__ mt_nctiles,
-             int32_t *const *__restrict__ plist,
-                 const uint32_t *__restrict__ pkey, const uint32_t *__restrict__ pcnt,
-                 const uint32_t *__restrict__ plast, const int32_t *__restrict__ pn,
-                 const int32_t *__restrict__ prev, int32_t *__restrict__ next, int32_t *__restrict__ dist_out,
-                 unsigned long long *__restrict__ counter, int64_t *__restrict__ ctr,
-                 const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
-    constexpr int BITS = 11, CAP = 1 << BITS;
-    __shared__ uint32_t skey[CAP], scnt[CAP], slast[CAP], slist[CAP];
-    __shared__ uint32_t nlist;
-    __shared__ int32_t bad;
-    __shared__ unsigned long long sred64[4];
+             int32_t *const *__restrict__ plist, const uint32_t *__restrict__ pkey, const uint32_t *__restrict__ pcnt,
+             const uint32_t *__restrict__ plast, const int32_t *__restrict__ pn, const int32_t *__restrict__ prev,
+             int32_t *__restrict__ next, int32_t *__restrict__ dist_out, unsigned long long *__restrict__ counter,
+             int64_t *__restrict__ ctr, const int32_t *__restrict__ stop) {
     if (*stop || ctr[kCtrCandMt] == 0) return;
+    const int lane = lane_id();
+    const int64_t wave = (int64_t)blockIdx.x * (256 / kWave) + (int64_t)(threadIdx.x / kWave);
+    const int64_t nw = (int64_t)gridDim.x * (256 / kWave);
     uint32_t changed = 0;
-    for (int s = threadIdx.x; s < CAP; s += 256) {
-        skey[s] = kEmpty;
-        scnt[s] = 0;
-        slast[s] = 0;
-    }
-    for (int64_t h = blockIdx.x; h < nmt; h += gridDim.x) {
-        if (!mt_cand[h]) continue;   // uniform per block
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            nlist = 0;
-            bad = 0;
-        }
-        __syncthreads();
+    for (int64_t h = wave; h < nmt; h += nw) {
+        if (!mt_cand[h]) continue;   // uniform per wave
         const int64_t t0 = mt_ctile0[h], nt = mt_nctiles[h];
-        for (int64_t t = t0 + threadIdx.x; t < t0 + nt; t += 256)
-            if (pn[t] < 0) bad = 1;
-        __syncthreads();
-        if (!bad) {
-            for (int64_t e = t0 * kCandK + threadIdx.x; e < (t0 + nt) * kCandK; e += 256) {
-                const int64_t t = e / kCandK;
-                if ((int)(e % kCandK) >= pn[t]) continue;
-                const uint32_t L = pkey[e];
-                uint32_t hs = hash_slot(L, BITS);
-                bool placed = false, created = false;
-                for (int probe = 0; probe < CAP; ++probe) {
-                    const uint32_t cur = skey[hs];
-                    if (cur == L) { placed = true; break; }
-                    if (cur == kEmpty) {
-                        const uint32_t o = atomicCAS(&skey[hs], kEmpty, L);
-                        if (o == kEmpty) { placed = created = true; break; }
-                        if (o == L) { placed = true; break; }
+        bool bad = false;
+        for (int64_t t = t0 + lane; t < t0 + nt; t += kWave) bad = bad || pn[t] < 0;
+        bad = __ballot(bad) != 0ull;
+        int32_t mkey = -1;   // merged label of this lane (-1: none)
+        uint32_t mcnt = 0, mlast = 0;
+        int nm = 0;          // merged labels so far (uniform)
+        for (int64_t e0 = t0 * kCandK; e0 < (t0 + nt) * kCandK && !bad; e0 += kWave) {
+            const int64_t e = e0 + lane;
+            const bool ok = e < (t0 + nt) * kCandK && (int)(e % kCandK) < pn[e / kCandK];
+            const int32_t ik = ok ? (int32_t)pkey[e] : -1;
+            const uint32_t ic = ok ? pcnt[e] : 0u, il = ok ? plast[e] : 0u;
+            unsigned long long ball = __ballot(ok);
+            while (ball) {
+                const int j = __ffsll((long long)ball) - 1;
+                ball &= ball - 1ull;
+                const int32_t k = __builtin_amdgcn_readlane(ik, j);
+                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)ic, j);
+                const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)il, j);
+                const unsigned long long hit = __ballot(mkey == k);
+                if (hit) {
+                    if (lane == __ffsll((long long)hit) - 1) {
+                        mcnt += c;
+                        mlast = max(mlast, l);
                     }
-                    hs = (hs + 1) & (CAP - 1);
+                } else if (nm == kWave) {
+                    bad = true;   // more labels than lanes
+                    break;
+                } else {
+                    if (lane == nm) {
+                        mkey = k;
+                        mcnt = c;
+                        mlast = l;
+                    }
+                    ++nm;
                 }
-                if (!placed) {   // more distinct labels than slots: plain path
-                    bad = 1;
-                    continue;
-                }
-                atomicAdd(&scnt[hs], pcnt[e]);
-                atomicMax(&slast[hs], plast[e]);
-                if (created) slist[atomicAdd(&nlist, 1u)] = hs;
             }
         }
-        __syncthreads();
-        const uint32_t nl = nlist;
-        unsigned long long best = 0;
-        uint32_t bestkey = 0;
-        for (uint32_t i = threadIdx.x; i < nl; i += 256) {
-            const uint32_t sl = slist[i];
-            const unsigned long long pk = ((unsigned long long)scnt[sl] << 32) | slast[sl];
-            if (pk > best) {
-                best = pk;
-                bestkey = skey[sl];
-            }
-        }
-        const unsigned long long top = block_max<4>(best, sred64);
-        __shared__ uint32_t swin;
-        if (best == top && top) swin = bestkey;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nl; i += 256) {
-            const uint32_t sl = slist[i];
-            skey[sl] = kEmpty;
-            scnt[sl] = 0;
-            slast[sl] = 0;
-        }
-        if (threadIdx.x == 0) {
-            const int64_t v = mt_vid[h];
-            if (bad) {
+        const int64_t v = mt_vid[h];
+        if (bad) {
+            if (lane == 0) {
                 if (h < nhubs) {
                     hub_sel[h] = 1;
                     atomicAdd((unsigned long long *)(ctr + kCtrPlainHubs), 1ull);
@@ -547,17 +522,20 @@ k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int6
                     plist[c][q] = (int32_t)v;
                 }
                 atomicAdd((unsigned long long *)(ctr + kCtrOverflow), 1ull);
-            } else {
-                const int32_t w = (int32_t)swin;
-                next[v] = w;
-                changed += (w != prev[v]);
-                if (dist_out) dist_out[v] = (int32_t)nl;
             }
+            continue;
+        }
+        const unsigned long long val = mkey >= 0 ? (((unsigned long long)mcnt << 32) | mlast) : 0ull;
+        const unsigned long long top = wave_max(val);
+        const unsigned long long wl = __ballot(mkey >= 0 && val == top);
+        const int32_t w = __builtin_amdgcn_readlane(mkey, __ffsll((long long)wl) - 1);
+        if (lane == 0 && wl) {
+            next[v] = w;
+            changed += (w != prev[v]);
+            if (dist_out) dist_out[v] = nm;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && changed) atomicAdd(counter, (unsigned long long)changed);
-    (void)err;
+    if (lane == 0 && changed) atomicAdd(counter, (unsigned long long)changed);
 }
 
 // Training-sample log: for every big vertex, (inputs from the previous
@@ -868,10 +846,10 @@ static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_
     prof_end(g, &pl);
     if (L->nmt && !g->ooc) {
         prof_begin(g, "cand_merge", 0.0, &pl);
-        k_cand_merge<E><<<grid_cap(g, L->nmt, 4), 256, 0, g->stream>>>(
+        k_cand_merge<E><<<grid_cap(g, (L->nmt + 3) / 4, 8), 256, 0, g->stream>>>(
             (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
             L->d_mt_nctiles, dpl, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, prev, next, L->distinct_out(g),
-            g->d_counter, L->d_plist_n, g->d_stop, g->d_err);
+            g->d_counter, L->d_plist_n, g->d_stop);
         prof_end(g, &pl);
     }
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "learned dispatch launch failed");
