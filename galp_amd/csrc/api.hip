@@ -157,6 +157,11 @@ static void dev_free_all(galp_graph *g) {
         if (pl.e0) (void)hipEventDestroy(pl.e0);
         if (pl.e1) (void)hipEventDestroy(pl.e1);
     }
+    for (int k = 0; k < 2; ++k) {
+        if (g->side[k]) (void)hipStreamDestroy(g->side[k]);
+        if (g->side_fork[k]) (void)hipEventDestroy(g->side_fork[k]);
+        if (g->side_join[k]) (void)hipEventDestroy(g->side_join[k]);
+    }
     if (g->stream) (void)hipStreamDestroy(g->stream);
 }
 
@@ -360,6 +365,12 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
 
     const bool ooc = g->opts.policy == GALP_POLICY_OOC_SYNC || g->opts.policy == GALP_POLICY_OOC_ASYNC;
     GALP_HIP(ctx, hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    if (!ooc)
+        for (int k = 0; k < 2; ++k) {
+            GALP_HIP(ctx, hipStreamCreateWithFlags(&g->side[k], hipStreamNonBlocking));
+            GALP_HIP(ctx, hipEventCreateWithFlags(&g->side_fork[k], hipEventDisableTiming));
+            GALP_HIP(ctx, hipEventCreateWithFlags(&g->side_join[k], hipEventDisableTiming));
+        }
     if (!ooc) GALP_TRY(dev_alloc(g, &g->d_nbr, m));
     GALP_TRY(dev_alloc(g, &g->d_lab[0], n));
     GALP_TRY(dev_alloc(g, &g->d_lab[1], n));
@@ -467,16 +478,40 @@ IterView full_view(const galp_graph *g) {
     return w;
 }
 
+bool concurrent_classes(const galp_graph *g) {
+    return !g->ooc && !g->prof.on && g->side[0] && g->side[1] && g->opts.variant != 0 &&
+           g->opts.policy != GALP_POLICY_LI;
+}
+
+int side_begin(galp_graph *g, int k, hipStream_t *main_stream) {
+    *main_stream = g->stream;
+    GALP_HIP(g->ctx, hipEventRecord(g->side_fork[k], g->stream));
+    GALP_HIP(g->ctx, hipStreamWaitEvent(g->side[k], g->side_fork[k], 0));
+    g->stream = g->side[k];
+    return GALP_OK;
+}
+
+int side_end(galp_graph *g, int k, hipStream_t main_stream) {
+    g->stream = main_stream;
+    GALP_HIP(g->ctx, hipEventRecord(g->side_join[k], g->side[k]));
+    return GALP_OK;
+}
+
+int side_join(galp_graph *g, int k) {
+    GALP_HIP(g->ctx, hipStreamWaitEvent(g->stream, g->side_join[k], 0));
+    return GALP_OK;
+}
+
 static int launch_iteration(galp_graph *g) {
     switch (g->opts.method) {
     case GALP_METHOD_HASH:
         if (g->opts.dispatch == GALP_DISPATCH_LEARNED) {
             if (!g->learn || !g->learn->H)
                 return fail(g->ctx, GALP_ESTATE, "learned dispatch needs a network: call galp_graph_set_mlp");
-            if (!g->ooc) GALP_TRY(launch_learned_front(g));   // out of core: per batch
         }
+        // out of core, the learned front runs per batch
         if (g->ooc) GALP_TRY(launch_ooc_iteration(g));
-        else GALP_TRY(launch_hash_iteration(g, full_view(g)));
+        else GALP_TRY(launch_hash_iteration_incore(g));
         if (g->learn) GALP_TRY(learn_after_iteration(g));
         break;
     case GALP_METHOD_SORT:

@@ -221,6 +221,11 @@ struct galp_graph {
     bool off64 = false;
     bool reorder = false;
     hipStream_t stream = nullptr;
+    // Side streams for the independent parts of an in-core iteration (the
+    // lane-group classes; the LDS classes), forked from and joined back to
+    // `stream` by events.
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t side_fork[2] = {nullptr, nullptr}, side_join[2] = {nullptr, nullptr};
 
     // CSR in HBM (internal numbering when reorder is on)
     void *d_off = nullptr;          // int32 or int64, n+1
@@ -287,6 +292,20 @@ int launch_hash_iteration(galp_graph *g, const IterView &w);
 int launch_hash_class(galp_graph *g, int b);
 // Iteration bookkeeping: records the change count, sets the stop flag.
 int launch_iter_end(galp_graph *g);
+// In-core iterations with independent class kernels on the side streams
+// (both methods); false where everything stays on the graph's stream: out of
+// core, with per-kernel profiling on, for the -l 0 and li layouts.
+bool concurrent_classes(const galp_graph *g);
+// Launches made between side_begin(g, k) and side_end(g, k, s) go to side
+// stream k, ordered after everything launched on the graph's stream so far;
+// side_begin returns the graph's stream, which side_end restores.
+// side_join(g, k) orders later launches on the graph's stream after them.
+int side_begin(galp_graph *g, int k, hipStream_t *main_stream);
+int side_end(galp_graph *g, int k, hipStream_t main_stream);
+int side_join(galp_graph *g, int k);
+// One in-core hash iteration with the learned front when dispatch is
+// learned, classes on side streams when concurrent_classes(g).
+int launch_hash_iteration_incore(galp_graph *g);
 
 // Sort method (lp_sort.hip).
 int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg);
@@ -320,7 +339,10 @@ int launch_batch_end(galp_graph *g);
 int learn_setup(galp_graph *g);
 void learn_free(galp_graph *g);
 int learn_reset_iteration(galp_graph *g);
-int launch_learned_front(galp_graph *g);
+// The dispatch and the candidate path; with merge = false the multi-tile
+// merge is left to launch_learned_merge.
+int launch_learned_front(galp_graph *g, bool merge = true);
+int launch_learned_merge(galp_graph *g);
 // Out-of-core: the dispatch and candidate path of one batch (vertices
 // [lo, hi), the batch's rebased offsets and neighbour buffer in w).
 int launch_learned_front_batch(galp_graph *g, const IterView &w, int64_t lo, int64_t hi);

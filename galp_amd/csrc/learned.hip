@@ -804,7 +804,8 @@ static void launch_dispatch(galp_graph *g, LearnState *L, int32_t *const *dpl, c
 // and lists are reset even when the range is empty, since the class kernels
 // that follow read them.
 template <typename E>
-static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_t v0, int64_t nv, bool first) {
+static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_t v0, int64_t nv, bool first,
+                         bool merge) {
     LearnState *L = g->learn;
     int32_t *const *dpl = L->d_plist;
     const int32_t *prev = g->d_lab[g->cur];
@@ -844,7 +845,7 @@ static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_
         off, nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, L->distinct_out(g), dpl,
         L->d_plist_n, hot, g->d_counter, g->d_stop);
     prof_end(g, &pl);
-    if (L->nmt && !g->ooc) {
+    if (merge && L->nmt && !g->ooc) {
         prof_begin(g, "cand_merge", 0.0, &pl);
         k_cand_merge<E><<<grid_cap(g, (L->nmt + 3) / 4, 8), 256, 0, g->stream>>>(
             (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
@@ -855,18 +856,37 @@ static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "learned dispatch launch failed");
 }
 
-int launch_learned_front(galp_graph *g) {
+int launch_learned_front(galp_graph *g, bool merge) {
     LearnState *L = g->learn;
-    return g->off64 ? learned_front<int64_t>(g, (const int64_t *)g->d_off, g->d_nbr, L->big_v0, L->big_n, true)
-                    : learned_front<int32_t>(g, (const int32_t *)g->d_off, g->d_nbr, L->big_v0, L->big_n, true);
+    return g->off64
+               ? learned_front<int64_t>(g, (const int64_t *)g->d_off, g->d_nbr, L->big_v0, L->big_n, true, merge)
+               : learned_front<int32_t>(g, (const int32_t *)g->d_off, g->d_nbr, L->big_v0, L->big_n, true, merge);
+}
+
+template <typename E>
+static int learned_merge(galp_graph *g) {
+    LearnState *L = g->learn;
+    if (!L->nmt || g->ooc || !L->front_n) return GALP_OK;
+    PendingLaunch pl;
+    prof_begin(g, "cand_merge", 0.0, &pl);
+    k_cand_merge<E><<<grid_cap(g, (L->nmt + 3) / 4, 8), 256, 0, g->stream>>>(
+        (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
+        L->d_mt_nctiles, L->d_plist, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, g->d_lab[g->cur],
+        g->d_lab[g->cur ^ 1], L->distinct_out(g), g->d_counter, L->d_plist_n, g->d_stop);
+    prof_end(g, &pl);
+    return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "candidate merge launch failed");
+}
+
+int launch_learned_merge(galp_graph *g) {
+    return g->off64 ? learned_merge<int64_t>(g) : learned_merge<int32_t>(g);
 }
 
 int launch_learned_front_batch(galp_graph *g, const IterView &w, int64_t lo, int64_t hi) {
     LearnState *L = g->learn;
     const int64_t v1 = std::min(hi, L->big_n);   // renumbered: big vertices are ids [0, big_n)
     const int64_t nv = lo < v1 ? v1 - lo : 0;
-    return g->off64 ? learned_front<int64_t>(g, (const int64_t *)w.off, w.nbr, lo, nv, lo == 0)
-                    : learned_front<int32_t>(g, (const int32_t *)w.off, w.nbr, lo, nv, lo == 0);
+    return g->off64 ? learned_front<int64_t>(g, (const int64_t *)w.off, w.nbr, lo, nv, lo == 0, true)
+                    : learned_front<int32_t>(g, (const int32_t *)w.off, w.nbr, lo, nv, lo == 0, true);
 }
 
 // After the iteration's LP kernels: log samples (if enabled), swap counts.

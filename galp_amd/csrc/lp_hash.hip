@@ -837,6 +837,50 @@ int launch_hash_iteration(galp_graph *g, const IterView &w) {
     return g->off64 ? hash_iteration<int64_t>(g, w) : hash_iteration<int32_t>(g, w);
 }
 
+// One in-core iteration.  Every class kernel reads prev and writes next only
+// at its own rows, so the classes are independent once their vertex lists
+// are final.  The lane-group classes (degree <= 64) go to side stream 0: at
+// the start of the iteration, or under learned dispatch after the dispatch
+// (which sets their resolver flag) and the candidate kernel (one
+// 1024-thread block per CU holding 152 KB of LDS, launched first so that it
+// finds its CUs free).  The LDS classes go to side stream 1 after the
+// candidate merge, which appends the rows it gives up to their lists; the
+// hub path stays on the graph's stream beside them.
+template <typename E>
+static int hash_iteration_incore(galp_graph *g) {
+    const IterView w = full_view(g);
+    const bool learned = g->opts.dispatch == GALP_DISPATCH_LEARNED;
+    if (!concurrent_classes(g)) {
+        if (learned) {
+            const int rc = launch_learned_front(g, true);
+            if (rc != GALP_OK) return rc;
+        }
+        return hash_iteration<E>(g, w);
+    }
+    const int32_t *prev = g->d_lab[g->cur];
+    int32_t *next = g->d_lab[g->cur ^ 1];
+    int rc = learned ? launch_learned_front(g, false) : GALP_OK;
+    hipStream_t ms = nullptr;
+    if (rc == GALP_OK) rc = side_begin(g, 0, &ms);
+    if (rc != GALP_OK) return rc;
+    for (int b = B_G64; b >= B_G4; --b) launch_class<E>(g, w, b, prev, next);
+    rc = side_end(g, 0, ms);
+    if (rc == GALP_OK && learned) rc = launch_learned_merge(g);
+    if (rc == GALP_OK) rc = side_begin(g, 1, &ms);
+    if (rc != GALP_OK) return rc;
+    for (int b = B_B4K; b >= B_W128; --b) launch_class<E>(g, w, b, prev, next);
+    rc = side_end(g, 1, ms);
+    if (rc != GALP_OK) return rc;
+    launch_class<E>(g, w, B_HUB, prev, next);
+    if ((rc = side_join(g, 0)) != GALP_OK || (rc = side_join(g, 1)) != GALP_OK) return rc;
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
+}
+
+int launch_hash_iteration_incore(galp_graph *g) {
+    return g->off64 ? hash_iteration_incore<int64_t>(g) : hash_iteration_incore<int32_t>(g);
+}
+
 int launch_iter_end(galp_graph *g) {
     k_iter_end<<<1, 1, 0, g->stream>>>(g->d_counter, g->d_stop, g->d_iter_counts, g->d_iters_done,
                                        g->iter % kMaxIters, (double)g->n * 1e-5);

@@ -993,10 +993,32 @@ int launch_sort_class(galp_graph *g, int b) {
     return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("class launch: ") + hipGetErrorString(e));
 }
 
+// In core, the classes are independent (each reads prev and writes next at
+// its own rows): the lane-group classes run on side stream 0, the LDS
+// classes on side stream 1, the hub path on the graph's stream.
+template <typename E>
+static int sort_iteration_concurrent(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next) {
+    hipStream_t ms = nullptr;
+    int rc = side_begin(g, 0, &ms);
+    if (rc != GALP_OK) return rc;
+    for (int b = B_G64; b >= B_G4 && rc == GALP_OK; --b) rc = sort_class<E>(g, w, b, prev, next);
+    const int rc0 = side_end(g, 0, ms);
+    if (rc != GALP_OK || rc0 != GALP_OK) return rc != GALP_OK ? rc : rc0;
+    if ((rc = side_begin(g, 1, &ms)) != GALP_OK) return rc;
+    for (int b = B_B4K; b >= B_W128 && rc == GALP_OK; --b) rc = sort_class<E>(g, w, b, prev, next);
+    const int rc1 = side_end(g, 1, ms);
+    if (rc != GALP_OK || rc1 != GALP_OK) return rc != GALP_OK ? rc : rc1;
+    if ((rc = sort_class<E>(g, w, B_HUB, prev, next)) != GALP_OK) return rc;
+    if ((rc = side_join(g, 0)) != GALP_OK || (rc = side_join(g, 1)) != GALP_OK) return rc;
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("sort kernel launch: ") + hipGetErrorString(e));
+}
+
 template <typename E>
 static int sort_iteration(galp_graph *g, const IterView &w) {
     const int32_t *prev = g->d_lab[g->cur];
     int32_t *next = g->d_lab[g->cur ^ 1];
+    if (!g->ooc && concurrent_classes(g)) return sort_iteration_concurrent<E>(g, w, prev, next);
     if (g->opts.policy == GALP_POLICY_LI) {
         // "load-imbalanced" ablation: every class <= 4096 through the
         // 512-thread block-per-vertex kernel, as the reference's li path
