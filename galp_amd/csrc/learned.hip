@@ -448,14 +448,19 @@ k_cand(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t
     if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
+constexpr int kMergeU = 8;   // entry batches per load round in k_cand_merge
+
 // Multi-tile vertices on the candidate path: merge the tiles' partial
 // candidates, one wave per vertex.  The merged labels live in lanes (lane j
-// holds label j with its summed count and largest position); the tiles'
-// entries are folded in one at a time, a ballot finding the lane of an equal
-// label.  A tile that overflowed, or more than 64 distinct labels in all,
-// hands the vertex to the plain path of its class (hubs: the plain hub
-// path), which runs after this kernel.  Counts add and positions take the
-// max, so the merged (count, last) of every label is exact.
+// holds label j), their summed counts and largest positions in the wave's
+// LDS slots.  The tiles' entries come 64 at a time: each finds its label
+// among the merged ones (a broadcast of each merged label), adds its count
+// and positions by LDS atomics; entries whose label is new are then placed
+// one at a time (a ballot finds an equal label placed meanwhile).  A tile
+// that overflowed, or more than 64 distinct labels in all, hands the vertex
+// to the plain path of its class (hubs: the plain hub path), which runs
+// after this kernel.  Counts add and positions take the max, so the merged
+// (count, last) of every label is exact.
 template <typename E>
 __global__ void __launch_bounds__(256)
 k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int64_t nmt, int64_t nhubs,
@@ -465,51 +470,88 @@ k_cand_merge(const E *__restrict__ off, const int32_t *__restrict__ mt_vid, int6
              const uint32_t *__restrict__ plast, const int32_t *__restrict__ pn, const int32_t *__restrict__ prev,
              int32_t *__restrict__ next, int32_t *__restrict__ dist_out, unsigned long long *__restrict__ counter,
              int64_t *__restrict__ ctr, const int32_t *__restrict__ stop) {
+    __shared__ uint32_t s_cnt[256 / kWave][kWave], s_last[256 / kWave][kWave];
     if (*stop || ctr[kCtrCandMt] == 0) return;
     const int lane = lane_id();
-    const int64_t wave = (int64_t)blockIdx.x * (256 / kWave) + (int64_t)(threadIdx.x / kWave);
+    const int wv = (int)(threadIdx.x / kWave);
+    uint32_t *scnt = s_cnt[wv], *slast = s_last[wv];
+    const int64_t wave = (int64_t)blockIdx.x * (256 / kWave) + wv;
     const int64_t nw = (int64_t)gridDim.x * (256 / kWave);
     uint32_t changed = 0;
     for (int64_t h = wave; h < nmt; h += nw) {
         if (!mt_cand[h]) continue;   // uniform per wave
         const int64_t t0 = mt_ctile0[h], nt = mt_nctiles[h];
+        const int64_t eend = (t0 + nt) * kCandK;
         bool bad = false;
-        for (int64_t t = t0 + lane; t < t0 + nt; t += kWave) bad = bad || pn[t] < 0;
-        bad = __ballot(bad) != 0ull;
         int32_t mkey = -1;   // merged label of this lane (-1: none)
-        uint32_t mcnt = 0, mlast = 0;
         int nm = 0;          // merged labels so far (uniform)
-        for (int64_t e0 = t0 * kCandK; e0 < (t0 + nt) * kCandK && !bad; e0 += kWave) {
-            const int64_t e = e0 + lane;
-            const bool ok = e < (t0 + nt) * kCandK && (int)(e % kCandK) < pn[e / kCandK];
-            const int32_t ik = ok ? (int32_t)pkey[e] : -1;
-            const uint32_t ic = ok ? pcnt[e] : 0u, il = ok ? plast[e] : 0u;
-            unsigned long long ball = __ballot(ok);
-            while (ball) {
-                const int j = __ffsll((long long)ball) - 1;
-                ball &= ball - 1ull;
-                const int32_t k = __builtin_amdgcn_readlane(ik, j);
-                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)ic, j);
-                const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)il, j);
-                const unsigned long long hit = __ballot(mkey == k);
-                if (hit) {
-                    if (lane == __ffsll((long long)hit) - 1) {
-                        mcnt += c;
-                        mlast = max(mlast, l);
+        scnt[lane] = 0u;
+        slast[lane] = 0u;
+        __builtin_amdgcn_wave_barrier();   // cleared before any lane adds
+        for (int64_t eb = t0 * kCandK; eb < eend && !bad; eb += kMergeU * kWave) {
+            // kMergeU batches of 64 entries in flight: one wave walks a
+            // hub's thousands of tiles, so it waits on load latency
+            int32_t bk[kMergeU];
+            uint32_t bc[kMergeU], bl[kMergeU];
+            int32_t bn[kMergeU];
+#pragma unroll
+            for (int u = 0; u < kMergeU; ++u) {
+                const int64_t e = eb + u * kWave + lane;
+                const int64_t ec = e < eend ? e : eend - 1;
+                bn[u] = pn[ec / kCandK];
+                bk[u] = (int32_t)pkey[ec];
+                bc[u] = pcnt[ec];
+                bl[u] = plast[ec];
+            }
+            bool over = false;   // a tile that overflowed
+#pragma unroll
+            for (int u = 0; u < kMergeU; ++u) over = over || (eb + u * kWave + lane < eend && bn[u] < 0);
+            if (__ballot(over)) {
+                bad = true;
+                break;
+            }
+#pragma unroll
+            for (int u = 0; u < kMergeU && !bad; ++u) {
+                const int64_t e = eb + u * kWave + lane;
+                const bool ok = e < eend && (int)(e % kCandK) < bn[u];
+                const int32_t ik = ok ? bk[u] : -1;
+                const uint32_t ic = bc[u], il = bl[u];
+                int slot = -1;
+                for (int m = 0; m < nm; ++m) slot = __builtin_amdgcn_readlane(mkey, m) == ik ? m : slot;
+                if (ok && slot >= 0) {
+                    atomicAdd(&scnt[slot], ic);
+                    atomicMax(&slast[slot], il);
+                }
+                unsigned long long fresh = __ballot(ok && slot < 0);
+                while (fresh) {
+                    const int j = __ffsll((long long)fresh) - 1;
+                    fresh &= fresh - 1ull;
+                    const int32_t k = __builtin_amdgcn_readlane(ik, j);
+                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)ic, j);
+                    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)il, j);
+                    const unsigned long long hit = __ballot(mkey == k);   // placed earlier in this batch
+                    if (hit) {
+                        if (lane == __ffsll((long long)hit) - 1) {
+                            atomicAdd(&scnt[lane], c);
+                            atomicMax(&slast[lane], l);
+                        }
+                    } else if (nm == kWave) {
+                        bad = true;   // more labels than lanes
+                        break;
+                    } else {
+                        if (lane == nm) {
+                            mkey = k;
+                            atomicAdd(&scnt[lane], c);
+                            atomicMax(&slast[lane], l);
+                        }
+                        ++nm;
                     }
-                } else if (nm == kWave) {
-                    bad = true;   // more labels than lanes
-                    break;
-                } else {
-                    if (lane == nm) {
-                        mkey = k;
-                        mcnt = c;
-                        mlast = l;
-                    }
-                    ++nm;
                 }
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t mcnt = scnt[lane], mlast = slast[lane];
+        __builtin_amdgcn_wave_barrier();   // read before the next vertex clears the slots
         const int64_t v = mt_vid[h];
         if (bad) {
             if (lane == 0) {
