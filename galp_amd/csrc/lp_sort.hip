@@ -14,8 +14,10 @@
 // where its rows fit:
 //   - degree <= 64: a bitonic sort of (label, position) across the group's
 //     lanes (register shuffles), then run ends and run starts from ballots;
-//   - 65..1024: a bitonic sort of (label, position) in LDS per vertex;
-//     1025..4096: an LDS LSD radix sort of the labels (8-bit digits, stable
+//   - 65..1024: a bitonic sort of (label, position) in one wave's
+//     registers per vertex (2..16 keys per lane; lane exchanges by DPP,
+//     ds_swizzle and permutes), the LDS bitonic network as the autotuner's
+//     alternative; 1025..4096: an LDS LSD radix sort of the labels (8-bit digits, stable
 //     multisplit by ballots, positions as payload); run lengths by binary
 //     search for each run's first element;
 //   - hubs (> 4096): a segmented LSD radix sort (8-bit digits, as many passes
@@ -193,6 +195,159 @@ k_sort_bitonic(const E *__restrict__ off, const int32_t *__restrict__ nbr, const
     pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
     __syncthreads();
     block_count_add<T / kWave>(pol.changed, counter, sred);
+}
+
+// ---------------------------------------------------------------------------
+// Rows of 65..1024 edges, one wave per vertex, sorted in registers.  Lane l
+// holds elements l*EPT .. l*EPT+EPT-1 of a bitonic network (Batcher 1968)
+// over N = 64*EPT keys (label << 32 | position in the row), padded with
+// all-ones.  A compare-exchange at distance j < EPT stays in the lane; a
+// farther one pairs lane l with lane l ^ (j / EPT) through DPP moves
+// (xor 1, 2, 4, 8), ds_swizzle (16) or a permute (32).  No LDS, no barrier.
+// Sorted, a run start is an element whose predecessor holds another label;
+// an exclusive max-scan over lanes of each lane's last start gives every run
+// end its run's first index, hence the count; the run's last element holds
+// its largest position (positions break ties in the key).  The wave max of
+// (count << 32 | last) picks the winner.
+template <int X>
+__device__ __forceinline__ unsigned long long lane_xor_u64(unsigned long long v) {
+    const uint32_t lo = (uint32_t)lane_xor<X>((int32_t)(uint32_t)v);
+    const uint32_t hi = (uint32_t)lane_xor<X>((int32_t)(uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Compare-exchange steps j = J, J/2, .., 1 of merge stage KK.
+template <int EPT, int KK, int J>
+__device__ __forceinline__ void wave_bitonic_steps(unsigned long long (&key)[EPT], int lane) {
+    if constexpr (J < EPT) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            if ((k & J) == 0) {
+                const int q = k | J;
+                const bool up = ((lane * EPT + k) & KK) == 0;
+                const unsigned long long a = key[k], b = key[q];
+                const bool sw = (a > b) == up;
+                key[k] = sw ? b : a;
+                key[q] = sw ? a : b;
+            }
+        }
+    } else {
+        constexpr int X = J / EPT;
+        const bool lower = (lane & X) == 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const unsigned long long a = key[k], p = lane_xor_u64<X>(a);
+            const bool up = ((lane * EPT + k) & KK) == 0;
+            const unsigned long long lo = a < p ? a : p, hi = a < p ? p : a;
+            key[k] = lower == up ? lo : hi;
+        }
+    }
+    if constexpr (J > 1) wave_bitonic_steps<EPT, KK, J / 2>(key, lane);
+}
+
+template <int EPT, int KK = 2>
+__device__ __forceinline__ void wave_bitonic(unsigned long long (&key)[EPT], int lane) {
+    wave_bitonic_steps<EPT, KK, KK / 2>(key, lane);
+    if constexpr (KK < kWave * EPT) wave_bitonic<EPT, KK * 2>(key, lane);
+}
+
+template <int EPT, typename E>
+struct WaveBitonicPolicy {
+    static constexpr int N = kWave * EPT;
+    using Item = typename LdsBitonicPolicy<kWave, EPT, E>::Item;
+    const E *off;
+    const int32_t *prev;
+    int32_t *next;
+    const int32_t *vlist;
+    int64_t v0, nv;
+    int32_t *err;
+    uint32_t changed;
+
+    __device__ Item item(int64_t i) const {
+        Item it{false, 0, 0, 0, 0};
+        if (i < nv) {
+            it.v = vlist ? (int64_t)vlist[v0 + i] : v0 + i;
+            it.b = (int64_t)off[it.v];
+            it.len = (int)((int64_t)off[it.v + 1] - it.b);
+            it.old = prev[it.v];
+            it.valid = true;
+            if (it.len > N) {   // never planned into this class
+                if (threadIdx.x == 0) atomicOr(err, 8);
+                it.len = 0;
+            }
+        }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.b; }
+
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        if (it.len == 0) return;
+        const int lane = lane_id();
+        unsigned long long key[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * kWave + lane;   // the pipeline's slot of lab[k]
+            key[k] = pos < it.len ? (((unsigned long long)(uint32_t)lab[k] << 32) | (uint32_t)pos) : ~0ull;
+        }
+        wave_bitonic<EPT>(key, lane);
+        constexpr uint32_t kPad = 0xffffffffu;   // labels are < 2^31
+        uint32_t L[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) L[k] = (uint32_t)(key[k] >> 32);
+        const uint32_t before = (uint32_t)__shfl_up((int)L[EPT - 1], 1);   // lane - 1's last element
+        const uint32_t after = (uint32_t)__shfl_down((int)L[0], 1);        // lane + 1's first element
+        int ls = -1;   // this lane's last run start
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int i = lane * EPT + k;
+            const uint32_t pl = k ? L[k - 1] : before;
+            if (L[k] != kPad && (i == 0 || pl != L[k])) ls = i;
+        }
+        int inc = ls;   // inclusive max-scan over lanes
+#pragma unroll
+        for (int sft = 1; sft < kWave; sft <<= 1) {
+            const int y = __shfl_up(inc, sft);
+            if (lane >= sft) inc = inc > y ? inc : y;
+        }
+        int cur = __shfl_up(inc, 1);   // start of the run open at my first element
+        if (lane == 0) cur = -1;
+        unsigned long long best = 0;
+        uint32_t blab = 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int i = lane * EPT + k;
+            const uint32_t pl = k ? L[k - 1] : before;
+            if (L[k] != kPad && (i == 0 || pl != L[k])) cur = i;
+            const uint32_t nl = k < EPT - 1 ? L[k + 1] : (lane == kWave - 1 ? kPad : after);
+            if (L[k] != kPad && nl != L[k]) {
+                const unsigned long long val =
+                    ((unsigned long long)(uint32_t)(i - cur + 1) << 32) | (uint32_t)key[k];
+                if (val > best) {
+                    best = val;
+                    blab = L[k];
+                }
+            }
+        }
+        const unsigned long long top = wave_max(best);
+        const unsigned long long wl = __ballot(best == top && best != 0ull);
+        const int32_t w = __builtin_amdgcn_readlane((int)blab, __ffsll((long long)wl) - 1);
+        if (lane == 0) {
+            next[it.v] = w;
+            changed += (w != it.old);
+        }
+    }
+};
+
+template <int EPT, typename E>
+__global__ void __launch_bounds__(kWave)
+k_sort_wave(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+            int32_t *__restrict__ next, const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+    __shared__ uint32_t sred[1];
+    if (*stop) return;
+    WaveBitonicPolicy<EPT, E> pol{off, prev, next, vlist, v0, nv, err, 0u};
+    pipelined<kWave, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    block_count_add<1>(pol.changed, counter, sred);
 }
 
 // ---------------------------------------------------------------------------
@@ -883,6 +1038,18 @@ static void sort_lds(galp_graph *g, const IterView &w, int bid, const int32_t *p
     prof_end(g, &pl);
 }
 
+template <int EPT, typename E>
+static void sort_wave(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
+                      int blocks_per_cu) {
+    const BucketRange &r = w.cls[bid];
+    if (!r.nv) return;
+    PendingLaunch pl;
+    prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
+    k_sort_wave<EPT, E><<<grid_for(g, r.nv, blocks_per_cu), kWave, 0, g->stream>>>(
+        (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
+    prof_end(g, &pl);
+}
+
 // Hubs [h0, h1) with tiles [t0, t1): the histogram, scan and scatter of every
 // pass cover just those tiles; the scan starts at the hubs' first compact
 // slot, so the output lands at the same places as in a whole-graph pass.
@@ -952,24 +1119,24 @@ static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *pr
         else sort_lds<256, 8, E, true>(g, w, b, prev, next, 4);
         break;
     case B_B1K:
-        if (c == 1) sort_lds<256, 4, E, false>(g, w, b, prev, next, 4);
+        if (c == 1) sort_lds<128, 8, E, false>(g, w, b, prev, next, 8);
         else if (c == 2) sort_lds<128, 8, E, true>(g, w, b, prev, next, 8);
-        else sort_lds<128, 8, E, false>(g, w, b, prev, next, 8);
+        else sort_wave<16, E>(g, w, b, prev, next, 16);
         break;
     case B_W512:
-        if (c == 1) sort_lds<128, 4, E, false>(g, w, b, prev, next, 8);
+        if (c == 1) sort_lds<64, 8, E, false>(g, w, b, prev, next, 16);
         else if (c == 2) sort_lds<64, 8, E, true>(g, w, b, prev, next, 16);
-        else sort_lds<64, 8, E, false>(g, w, b, prev, next, 16);
+        else sort_wave<8, E>(g, w, b, prev, next, 16);
         break;
     case B_W256:
-        if (c == 1) sort_lds<128, 2, E, false>(g, w, b, prev, next, 12);
+        if (c == 1) sort_lds<64, 4, E, false>(g, w, b, prev, next, 24);
         else if (c == 2) sort_lds<64, 4, E, true>(g, w, b, prev, next, 24);
-        else sort_lds<64, 4, E, false>(g, w, b, prev, next, 24);
+        else sort_wave<4, E>(g, w, b, prev, next, 24);
         break;
     case B_W128:
-        if (c == 1) sort_lds<64, 2, E, false>(g, w, b, prev, next, 16);
+        if (c == 1) sort_lds<64, 2, E, false>(g, w, b, prev, next, 32);
         else if (c == 2) sort_lds<64, 2, E, true>(g, w, b, prev, next, 32);
-        else sort_lds<64, 2, E, false>(g, w, b, prev, next, 32);
+        else sort_wave<2, E>(g, w, b, prev, next, 32);
         break;
     default: {
         const int bpc = c == 1 ? 16 : c == 2 ? 4 : 8;
