@@ -198,17 +198,19 @@ k_sort_bitonic(const E *__restrict__ off, const int32_t *__restrict__ nbr, const
 }
 
 // ---------------------------------------------------------------------------
-// Rows of 65..1024 edges, one wave per vertex, sorted in registers.  Lane l
-// holds elements l*EPT .. l*EPT+EPT-1 of a bitonic network (Batcher 1968)
-// over N = 64*EPT keys (label << 32 | position in the row), padded with
-// all-ones.  A compare-exchange at distance j < EPT stays in the lane; a
-// farther one pairs lane l with lane l ^ (j / EPT) through DPP moves
-// (xor 1, 2, 4, 8), ds_swizzle (16) or a permute (32).  No LDS, no barrier.
-// Sorted, a run start is an element whose predecessor holds another label;
-// an exclusive max-scan over lanes of each lane's last start gives every run
-// end its run's first index, hence the count; the run's last element holds
-// its largest position (positions break ties in the key).  The wave max of
-// (count << 32 | last) picks the winner.
+// Rows of 65..4096 edges sorted in registers, one block of NW waves per
+// vertex.  Thread t holds elements t*EPT .. t*EPT+EPT-1 of a bitonic network
+// (Batcher 1968) over N = 64*NW*EPT keys (label << 32 | position in the
+// row), padded with all-ones.  A compare-exchange at distance j < EPT stays
+// in the thread; up to 64*EPT it pairs lane l with lane l ^ (j / EPT) of the
+// same wave through DPP moves (xor 1, 2, 4, 8), ds_swizzle (16) or a permute
+// (32); only the distances that cross waves (the last one or two of each of
+// the final merge stages) go through LDS.  Sorted, a run start is an
+// element whose predecessor holds another label; an exclusive max-scan of
+// each thread's last start gives every run end its run's first index, hence
+// the count; the run's last element holds its largest position (positions
+// break ties in the key).  The block max of (count << 32 | last) picks the
+// winner.
 template <int X>
 __device__ __forceinline__ unsigned long long lane_xor_u64(unsigned long long v) {
     const uint32_t lo = (uint32_t)lane_xor<X>((int32_t)(uint32_t)v);
@@ -216,15 +218,18 @@ __device__ __forceinline__ unsigned long long lane_xor_u64(unsigned long long v)
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Compare-exchange steps j = J, J/2, .., 1 of merge stage KK.
-template <int EPT, int KK, int J>
-__device__ __forceinline__ void wave_bitonic_steps(unsigned long long (&key)[EPT], int lane) {
+// Compare-exchange steps j = J, J/2, .., 1 of merge stage KK; sx: NW*64*EPT
+// words of LDS for the steps that cross waves.
+template <int NW, int EPT, int KK, int J>
+__device__ __forceinline__ void reg_bitonic_steps(unsigned long long (&key)[EPT], int tid,
+                                                  unsigned long long *sx) {
+    constexpr int T = NW * kWave;
     if constexpr (J < EPT) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             if ((k & J) == 0) {
                 const int q = k | J;
-                const bool up = ((lane * EPT + k) & KK) == 0;
+                const bool up = ((tid * EPT + k) & KK) == 0;
                 const unsigned long long a = key[k], b = key[q];
                 const bool sw = (a > b) == up;
                 key[k] = sw ? b : a;
@@ -232,34 +237,48 @@ __device__ __forceinline__ void wave_bitonic_steps(unsigned long long (&key)[EPT
             }
         }
     } else {
-        constexpr int X = J / EPT;
-        const bool lower = (lane & X) == 0;
+        constexpr int X = J / EPT;   // the partner thread is tid ^ X
+        const bool lower = (tid & X) == 0;
+        if constexpr (X >= kWave) {
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) sx[k * T + tid] = key[k];
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const unsigned long long a = key[k], p = lane_xor_u64<X>(a);
-            const bool up = ((lane * EPT + k) & KK) == 0;
+            const unsigned long long a = key[k];
+            unsigned long long p;
+            if constexpr (X >= kWave) p = sx[k * T + (tid ^ X)];
+            else p = lane_xor_u64<X>(a);
+            const bool up = ((tid * EPT + k) & KK) == 0;
             const unsigned long long lo = a < p ? a : p, hi = a < p ? p : a;
             key[k] = lower == up ? lo : hi;
         }
+        if constexpr (X >= kWave) __syncthreads();   // read before the next write
     }
-    if constexpr (J > 1) wave_bitonic_steps<EPT, KK, J / 2>(key, lane);
+    if constexpr (J > 1) reg_bitonic_steps<NW, EPT, KK, J / 2>(key, tid, sx);
 }
 
-template <int EPT, int KK = 2>
-__device__ __forceinline__ void wave_bitonic(unsigned long long (&key)[EPT], int lane) {
-    wave_bitonic_steps<EPT, KK, KK / 2>(key, lane);
-    if constexpr (KK < kWave * EPT) wave_bitonic<EPT, KK * 2>(key, lane);
+template <int NW, int EPT, int KK = 2>
+__device__ __forceinline__ void reg_bitonic(unsigned long long (&key)[EPT], int tid, unsigned long long *sx) {
+    reg_bitonic_steps<NW, EPT, KK, KK / 2>(key, tid, sx);
+    if constexpr (KK < NW * kWave * EPT) reg_bitonic<NW, EPT, KK * 2>(key, tid, sx);
 }
 
-template <int EPT, typename E>
-struct WaveBitonicPolicy {
-    static constexpr int N = kWave * EPT;
-    using Item = typename LdsBitonicPolicy<kWave, EPT, E>::Item;
+template <int NW, int EPT, typename E>
+struct RegBitonicPolicy {
+    static constexpr int T = NW * kWave;
+    static constexpr int N = T * EPT;
+    using Item = typename LdsBitonicPolicy<T, EPT, E>::Item;
     const E *off;
     const int32_t *prev;
     int32_t *next;
     const int32_t *vlist;
     int64_t v0, nv;
+    unsigned long long *sx;   // N words (NW > 1)
+    uint32_t *sedge;          // 2*T: each thread's first and last label (NW > 1)
+    int *sw;                  // NW
+    unsigned long long *sred64;   // NW
     int32_t *err;
     uint32_t changed;
 
@@ -281,29 +300,39 @@ struct WaveBitonicPolicy {
     __device__ int64_t edge0(const Item &it) const { return it.b; }
 
     __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
-        if (it.len == 0) return;
+        if (it.len == 0) return;   // uniform per block
+        const int tid = (int)threadIdx.x;
         const int lane = lane_id();
         unsigned long long key[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const int pos = k * kWave + lane;   // the pipeline's slot of lab[k]
+            const int pos = k * T + tid;   // the pipeline's slot of lab[k]
             key[k] = pos < it.len ? (((unsigned long long)(uint32_t)lab[k] << 32) | (uint32_t)pos) : ~0ull;
         }
-        wave_bitonic<EPT>(key, lane);
+        reg_bitonic<NW, EPT>(key, tid, sx);
         constexpr uint32_t kPad = 0xffffffffu;   // labels are < 2^31
         uint32_t L[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) L[k] = (uint32_t)(key[k] >> 32);
-        const uint32_t before = (uint32_t)__shfl_up((int)L[EPT - 1], 1);   // lane - 1's last element
-        const uint32_t after = (uint32_t)__shfl_down((int)L[0], 1);        // lane + 1's first element
-        int ls = -1;   // this lane's last run start
+        uint32_t before, after;   // thread tid - 1's last element, tid + 1's first
+        if constexpr (NW == 1) {
+            before = (uint32_t)__shfl_up((int)L[EPT - 1], 1);
+            after = lane == kWave - 1 ? kPad : (uint32_t)__shfl_down((int)L[0], 1);
+        } else {
+            sedge[tid] = L[0];
+            sedge[T + tid] = L[EPT - 1];
+            __syncthreads();
+            before = tid ? sedge[T + tid - 1] : kPad;
+            after = tid < T - 1 ? sedge[tid + 1] : kPad;
+        }
+        int ls = -1;   // this thread's last run start
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const int i = lane * EPT + k;
+            const int i = tid * EPT + k;
             const uint32_t pl = k ? L[k - 1] : before;
             if (L[k] != kPad && (i == 0 || pl != L[k])) ls = i;
         }
-        int inc = ls;   // inclusive max-scan over lanes
+        int inc = ls;   // inclusive max-scan over the wave's lanes
 #pragma unroll
         for (int sft = 1; sft < kWave; sft <<= 1) {
             const int y = __shfl_up(inc, sft);
@@ -311,14 +340,22 @@ struct WaveBitonicPolicy {
         }
         int cur = __shfl_up(inc, 1);   // start of the run open at my first element
         if (lane == 0) cur = -1;
+        if constexpr (NW > 1) {   // and over the earlier waves
+            const int w = tid / kWave;
+            if (lane == kWave - 1) sw[w] = inc;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+                if (q < w) cur = cur > sw[q] ? cur : sw[q];
+        }
         unsigned long long best = 0;
         uint32_t blab = 0;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const int i = lane * EPT + k;
+            const int i = tid * EPT + k;
             const uint32_t pl = k ? L[k - 1] : before;
             if (L[k] != kPad && (i == 0 || pl != L[k])) cur = i;
-            const uint32_t nl = k < EPT - 1 ? L[k + 1] : (lane == kWave - 1 ? kPad : after);
+            const uint32_t nl = k < EPT - 1 ? L[k + 1] : after;
             if (L[k] != kPad && nl != L[k]) {
                 const unsigned long long val =
                     ((unsigned long long)(uint32_t)(i - cur + 1) << 32) | (uint32_t)key[k];
@@ -328,26 +365,40 @@ struct WaveBitonicPolicy {
                 }
             }
         }
-        const unsigned long long top = wave_max(best);
-        const unsigned long long wl = __ballot(best == top && best != 0ull);
-        const int32_t w = __builtin_amdgcn_readlane((int)blab, __ffsll((long long)wl) - 1);
-        if (lane == 0) {
-            next[it.v] = w;
-            changed += (w != it.old);
+        if constexpr (NW == 1) {
+            const unsigned long long top = wave_max(best);
+            const unsigned long long wl = __ballot(best == top && best != 0ull);
+            const int32_t w = __builtin_amdgcn_readlane((int)blab, __ffsll((long long)wl) - 1);
+            if (lane == 0) {
+                next[it.v] = w;
+                changed += (w != it.old);
+            }
+        } else {
+            const unsigned long long top = block_max<NW>(best, sred64);   // barriers inside
+            if (best == top && best != 0ull) {   // unique: positions are distinct
+                next[it.v] = (int32_t)blab;
+                changed += ((int32_t)blab != it.old);
+            }
         }
     }
 };
 
-template <int EPT, typename E>
-__global__ void __launch_bounds__(kWave)
-k_sort_wave(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
-            int32_t *__restrict__ next, const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
-            unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
-    __shared__ uint32_t sred[1];
+template <int NW, int EPT, typename E>
+__global__ void __launch_bounds__(NW * kWave)
+k_sort_reg(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+           int32_t *__restrict__ next, const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+           unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop, int32_t *__restrict__ err) {
+    constexpr int T = NW * kWave;
+    __shared__ unsigned long long sx[NW > 1 ? T * EPT : 1];
+    __shared__ uint32_t sedge[2 * T];
+    __shared__ int sw[NW];
+    __shared__ unsigned long long sred64[NW];
+    __shared__ uint32_t sred[NW];
     if (*stop) return;
-    WaveBitonicPolicy<EPT, E> pol{off, prev, next, vlist, v0, nv, err, 0u};
-    pipelined<kWave, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
-    block_count_add<1>(pol.changed, counter, sred);
+    RegBitonicPolicy<NW, EPT, E> pol{off, prev, next, vlist, v0, nv, sx, sedge, sw, sred64, err, 0u};
+    pipelined<T, EPT>(pol, nbr, prev, (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    if constexpr (NW > 1) __syncthreads();
+    block_count_add<NW>(pol.changed, counter, sred);
 }
 
 // ---------------------------------------------------------------------------
@@ -1038,14 +1089,14 @@ static void sort_lds(galp_graph *g, const IterView &w, int bid, const int32_t *p
     prof_end(g, &pl);
 }
 
-template <int EPT, typename E>
-static void sort_wave(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
-                      int blocks_per_cu) {
+template <int NW, int EPT, typename E>
+static void sort_reg(galp_graph *g, const IterView &w, int bid, const int32_t *prev, int32_t *next,
+                     int blocks_per_cu) {
     const BucketRange &r = w.cls[bid];
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
-    k_sort_wave<EPT, E><<<grid_for(g, r.nv, blocks_per_cu), kWave, 0, g->stream>>>(
+    k_sort_reg<NW, EPT, E><<<grid_for(g, r.nv, blocks_per_cu), NW * kWave, 0, g->stream>>>(
         (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, g->d_err);
     prof_end(g, &pl);
 }
@@ -1109,7 +1160,7 @@ static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *pr
         if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 4 : c == 2 ? 16 : 8);
         break;
     case B_B4K:
-        if (c == 1) sort_lds<1024, 4, E, true>(g, w, b, prev, next, 1);
+        if (c == 1) sort_reg<4, 16, E>(g, w, b, prev, next, 2);
         else if (c == 2) sort_lds<256, 16, E, true>(g, w, b, prev, next, 4);
         else sort_lds<512, 8, E, true>(g, w, b, prev, next, 2);
         break;
@@ -1121,22 +1172,22 @@ static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *pr
     case B_B1K:
         if (c == 1) sort_lds<128, 8, E, false>(g, w, b, prev, next, 8);
         else if (c == 2) sort_lds<128, 8, E, true>(g, w, b, prev, next, 8);
-        else sort_wave<16, E>(g, w, b, prev, next, 16);
+        else sort_reg<1, 16, E>(g, w, b, prev, next, 16);
         break;
     case B_W512:
         if (c == 1) sort_lds<64, 8, E, false>(g, w, b, prev, next, 16);
         else if (c == 2) sort_lds<64, 8, E, true>(g, w, b, prev, next, 16);
-        else sort_wave<8, E>(g, w, b, prev, next, 16);
+        else sort_reg<1, 8, E>(g, w, b, prev, next, 16);
         break;
     case B_W256:
         if (c == 1) sort_lds<64, 4, E, false>(g, w, b, prev, next, 24);
         else if (c == 2) sort_lds<64, 4, E, true>(g, w, b, prev, next, 24);
-        else sort_wave<4, E>(g, w, b, prev, next, 24);
+        else sort_reg<1, 4, E>(g, w, b, prev, next, 24);
         break;
     case B_W128:
         if (c == 1) sort_lds<64, 2, E, false>(g, w, b, prev, next, 32);
         else if (c == 2) sort_lds<64, 2, E, true>(g, w, b, prev, next, 32);
-        else sort_wave<2, E>(g, w, b, prev, next, 32);
+        else sort_reg<1, 2, E>(g, w, b, prev, next, 32);
         break;
     default: {
         const int bpc = c == 1 ? 16 : c == 2 ? 4 : 8;
