@@ -480,7 +480,7 @@ IterView full_view(const galp_graph *g) {
 
 bool concurrent_classes(const galp_graph *g) {
     return !g->ooc && !g->prof.on && g->side[0] && g->side[1] && g->opts.variant != 0 &&
-           g->opts.policy != GALP_POLICY_LI;
+           g->opts.policy != GALP_POLICY_LI && g->m >= kConcurrentMinEdges;
 }
 
 int side_begin(galp_graph *g, int k, hipStream_t *main_stream) {

@@ -294,7 +294,11 @@ int launch_hash_class(galp_graph *g, int b);
 int launch_iter_end(galp_graph *g);
 // In-core iterations with independent class kernels on the side streams
 // (both methods); false where everything stays on the graph's stream: out of
-// core, with per-kernel profiling on, for the -l 0 and li layouts.
+// core, with per-kernel profiling on, for the -l 0 and li layouts, and below
+// kConcurrentMinEdges, where an iteration's kernels are short and the fork
+// and join (two events and two stream waits per side stream) cost more than
+// the overlap gains (a 20000-vertex graph ran at half speed with them).
+constexpr int64_t kConcurrentMinEdges = (int64_t)1 << 24;
 bool concurrent_classes(const galp_graph *g);
 // Launches made between side_begin(g, k) and side_end(g, k, s) go to side
 // stream k, ordered after everything launched on the graph's stream so far;

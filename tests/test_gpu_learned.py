@@ -319,3 +319,17 @@ def test_learned_needs_a_network(ctx):
         with pytest.raises(_abi.GalpError) as e:
             g.set_mlp(12, np.zeros(_abi.mlp_num_params(12), np.float32), 1.0)
         assert e.value.code == -1
+
+
+@pytest.mark.parametrize("net", ["all-cand", "rand"])
+def test_learned_rmat20_concurrent_classes(ctx, net):
+    """Learned dispatch at RMAT scale 20 (m = 33.5M), where the lane-group and
+    LDS classes run on side streams beside the candidate and hub paths:
+    every iteration bit-exact against the oracle."""
+    off, nbr = _abi.gen_rmat(20, 16, seed=21)
+    assert len(nbr) >= 1 << 24
+    if net == "all-cand":
+        routed, _ = run_learned(ctx, off, nbr, 6, "rmat20-all-cand", const_net(8, 0.0), 8, 4.0)
+        assert routed > 0
+    else:
+        run_learned(ctx, off, nbr, 6, "rmat20-rand", _abi.mlp_init(32, 7), 32, 3.0)

@@ -269,3 +269,17 @@ def test_rmat22_full_size(ctx, method):
     bit-exact against the oracle."""
     off, nbr = _abi.gen_rmat(22, 16, seed=22)
     check_per_iteration(ctx, off, nbr, 10, "rmat22", method=method)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_rmat20_concurrent_classes(ctx, method):
+    """RMAT scale 20 (m = 33.5M, above the size from which the class kernels of
+    an iteration run on three streams): every iteration bit-exact against the
+    oracle, and a captured 10-iteration run gives the same final labels."""
+    off, nbr = _abi.gen_rmat(20, 16, seed=20)
+    assert len(nbr) >= 1 << 24
+    final = check_per_iteration(ctx, off, nbr, 10, "rmat20-streams", method=method)
+    with _abi.Graph(ctx, off, nbr, method=method, graph_capture=1) as g:
+        for _ in range(3):   # capture, then replays
+            labels, _ = g.run(10)
+            assert np.array_equal(labels, final)
