@@ -345,6 +345,7 @@ class Graph:
         out = np.zeros(16, np.int32)
         self.ctx.check(lib().galp_graph_autotune(self.h, reps, os.fsencode(cache_path) if cache_path else None,
                                                  out.ctypes.data))
+        self.tuned_streams = int(out[12])   # 1 or 3 streams per iteration (0: not timed)
         return out[:12]
 
     def set_tuning(self, options):

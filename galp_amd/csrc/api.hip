@@ -478,9 +478,14 @@ IterView full_view(const galp_graph *g) {
     return w;
 }
 
-bool concurrent_classes(const galp_graph *g) {
+bool concurrent_allowed(const galp_graph *g) {
     return !g->ooc && !g->prof.on && g->side[0] && g->side[1] && g->opts.variant != 0 &&
-           g->opts.policy != GALP_POLICY_LI && g->m >= kConcurrentMinEdges;
+           g->opts.policy != GALP_POLICY_LI;
+}
+
+bool concurrent_classes(const galp_graph *g) {
+    if (!concurrent_allowed(g) || g->streams == 1) return false;
+    return g->streams == 3 || (g->m >= kConcurrentMinEdges && g->n <= kConcurrentMaxVertices);
 }
 
 int side_begin(galp_graph *g, int k, hipStream_t *main_stream) {

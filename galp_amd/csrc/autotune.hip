@@ -9,7 +9,11 @@
 // merge workgroup sizes) and sort_class in lp_sort.hip (workgroup size x
 // keys per thread, radix or bitonic, grids).  The choice never changes
 // labels (every configuration computes the same rule), which is how it is
-// tested.  Choices can be cached in a text file, one line per class:
+// tested.  Last, where an iteration may run its class kernels on three
+// streams, whole iterations are timed on one stream and on three and the
+// faster is kept (the overlap helps while the label array stays in the
+// Infinity Cache and can hurt once it does not).  Choices can be cached in a
+// text file, one line per class (class 12 = the stream choice, 1 or 3):
 //   <arch> <method> <class> <log2 vertices> <log2 edges> <option>
 // and are reused for a graph whose class has the same arch, method and sizes.
 
@@ -25,6 +29,13 @@
 namespace galp {
 
 constexpr int kTuneOptions = 3;
+constexpr int kStreamsLine = NUM_BUCKETS;   // the cache line of the stream choice
+
+// The LP kernels of one in-core iteration (no bookkeeping kernel).
+static int one_iteration(galp_graph *g) {
+    return g->opts.method == GALP_METHOD_HASH ? launch_hash_iteration_incore(g)
+                                              : launch_sort_iteration(g, full_view(g));
+}
 
 static int ilog2(int64_t x) { return x > 0 ? 63 - __builtin_clzll((unsigned long long)x) : -1; }
 
@@ -41,8 +52,9 @@ static std::vector<CacheLine> read_cache(const char *path) {
     while (std::getline(f, line)) {
         std::istringstream is(line);
         CacheLine c;
-        if (is >> c.arch >> c.method >> c.cls >> c.lv >> c.le >> c.opt && c.cls >= 0 && c.cls < NUM_BUCKETS &&
-            c.opt >= 0 && c.opt < kTuneOptions)
+        if (!(is >> c.arch >> c.method >> c.cls >> c.lv >> c.le >> c.opt)) continue;
+        if ((c.cls >= 0 && c.cls < NUM_BUCKETS && c.opt >= 0 && c.opt < kTuneOptions) ||
+            (c.cls == kStreamsLine && (c.opt == 1 || c.opt == 3)))
             out.push_back(c);
     }
     return out;
@@ -115,6 +127,29 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
         g->tune[b] = (int8_t)bo;
         fresh.push_back(CacheLine{arch, g->opts.method, b, lv, le, bo});
     }
+    if (rc == GALP_OK && concurrent_allowed(g)) {   // whole iterations: one stream or three
+        const int lv = ilog2(g->n), le = ilog2(g->m);
+        int mode = 0;
+        for (const CacheLine &c : cache)
+            if (c.arch == arch && c.method == g->opts.method && c.cls == kStreamsLine && c.lv == lv && c.le == le)
+                mode = c.opt;
+        if (!mode) {
+            float t[2] = {0.f, 0.f};
+            for (int q = 0; q < 2 && rc == GALP_OK; ++q) {
+                g->streams = (int8_t)(q ? 3 : 1);
+                rc = one_iteration(g);   // warm-up
+                if (rc != GALP_OK) break;
+                (void)hipEventRecord(e0, g->stream);
+                for (int k = 0; k < reps && rc == GALP_OK; ++k) rc = one_iteration(g);
+                (void)hipEventRecord(e1, g->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = fail(g->ctx, GALP_EHIP, "autotune timing");
+                if (rc == GALP_OK) (void)hipEventElapsedTime(&t[q], e0, e1);
+            }
+            mode = t[1] < t[0] ? 3 : 1;
+            if (rc == GALP_OK) fresh.push_back(CacheLine{arch, g->opts.method, kStreamsLine, lv, le, mode});
+        }
+        g->streams = (int8_t)mode;
+    }
     g->prof.on = prof;
     g->opts.dispatch = dispatch;
     (void)hipEventDestroy(e0);
@@ -129,8 +164,11 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
         for (const CacheLine &c : fresh)
             f << c.arch << ' ' << c.method << ' ' << c.cls << ' ' << c.lv << ' ' << c.le << ' ' << c.opt << '\n';
     }
-    if (chosen)
+    if (chosen) {
         for (int b = 0; b < NUM_BUCKETS; ++b) chosen[b] = g->tune[b];
+        chosen[kStreamsLine] = g->streams;
+        for (int b = kStreamsLine + 1; b < 16; ++b) chosen[b] = 0;
+    }
     return GALP_OK;
 }
 

@@ -271,6 +271,7 @@ struct galp_graph {
     galp::OocState *ooc = nullptr;
     galp::BatchState *batch = nullptr;
     int8_t tune[galp::NUM_BUCKETS] = {};   // kernel configuration per degree class (autotuner)
+    int8_t streams = 0;   // in-core class kernels: 0 by size, 1 one stream, 3 three (the autotuner's choice)
     hipGraphExec_t gexec = nullptr;    // captured iteration loop (opts.graph_capture)
     int gexec_niter = -1;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
@@ -293,12 +294,17 @@ int launch_hash_class(galp_graph *g, int b);
 // Iteration bookkeeping: records the change count, sets the stop flag.
 int launch_iter_end(galp_graph *g);
 // In-core iterations with independent class kernels on the side streams
-// (both methods); false where everything stays on the graph's stream: out of
-// core, with per-kernel profiling on, for the -l 0 and li layouts, and below
-// kConcurrentMinEdges, where an iteration's kernels are short and the fork
-// and join (two events and two stream waits per side stream) cost more than
-// the overlap gains (a 20000-vertex graph ran at half speed with them).
+// (both methods).  Never out of core, with per-kernel profiling on, or for
+// the -l 0 and li layouts; otherwise as the autotuner chose (g->streams), or
+// by size: from kConcurrentMinEdges edges (below, an iteration's kernels are
+// short and the fork and join cost more than the overlap gains: a
+// 20000-vertex graph ran at half speed with them) while the label array
+// fits in half of the 256 MB Infinity Cache (kConcurrentMaxVertices: at
+// RMAT-26, 268 MB of labels, overlapping classes ran 3-6 % slower, at
+// RMAT-22 and LFR-8M 2-7 % faster).
 constexpr int64_t kConcurrentMinEdges = (int64_t)1 << 24;
+constexpr int64_t kConcurrentMaxVertices = (int64_t)1 << 25;
+bool concurrent_allowed(const galp_graph *g);
 bool concurrent_classes(const galp_graph *g);
 // Launches made between side_begin(g, k) and side_end(g, k, s) go to side
 // stream k, ordered after everything launched on the graph's stream so far;

@@ -231,16 +231,19 @@ GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *
 GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max, int32_t *count);
 
 /* ---- autotuner -------------------------------------------------------------
- * Each degree class of method 1 has kTuneOptions = 3 kernel configurations
- * (0 = default; the others change the workgroup size, slots per lane or
- * blocks per CU).  galp_graph_autotune times every option of every class
- * with vertices on labels[i] = i (`reps` launches each), keeps the fastest,
- * and leaves the graph initialised (labels[i] = i).  cache_path (may be
- * NULL) names a text file of earlier choices keyed by device arch and the
- * class's log2 sizes; hits are reused without timing and new choices are
- * appended.  chosen (may be NULL) receives 16 entries.  Labels never depend
- * on the options.  No reference counterpart (fixed block sizes,
- * lfht.cuh:127-165). */
+ * Each degree class of either method has kTuneOptions = 3 kernel
+ * configurations (0 = default; the others change the workgroup size, slots
+ * or keys per lane, the sort algorithm or blocks per CU).
+ * galp_graph_autotune times every option of every class with vertices on
+ * labels[i] = i (`reps` launches each) and keeps the fastest; then, in core,
+ * it times whole iterations with the class kernels on one stream and on
+ * three and keeps the faster.  It leaves the graph initialised
+ * (labels[i] = i).  cache_path (may be NULL) names a text file of earlier
+ * choices keyed by device arch, method and the class's log2 sizes; hits are
+ * reused without timing and new choices are appended.  chosen (may be NULL)
+ * receives 16 entries: the 12 class options, then the stream choice (1 or 3;
+ * 0 where not timed), then zeros.  Labels never depend on the choices.  No
+ * reference counterpart (fixed block sizes, lfht.cuh:127-165). */
 GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_path, int32_t *chosen);
 /* Sets the option of every class directly (12 entries, each 0..2). */
 GALP_API int galp_graph_set_tuning(galp_graph *g, const int32_t *options);

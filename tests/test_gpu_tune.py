@@ -92,3 +92,29 @@ def test_autotune_refuses_other_policies(ctx):
         with pytest.raises(_abi.GalpError) as e:
             g.autotune(reps=1)
         assert e.value.code == -4
+
+
+@pytest.mark.parametrize("method", [_abi.METHOD_HASH, _abi.METHOD_SORT])
+def test_autotune_times_streams(ctx, tmp_path, method):
+    """The autotuner also picks one stream or three for whole iterations; either
+    choice, from the cache or timed, gives the oracle's labels."""
+    off, nbr = _abi.gen_rmat(16, 16, seed=25)
+    final, _, _ = oracle.lp_run(off, nbr, niter=10)
+    cache = str(tmp_path / "tune.txt")
+    with _abi.Graph(ctx, off, nbr, method=method) as g:
+        g.autotune(reps=2, cache_path=cache)
+        assert g.tuned_streams in (1, 3)
+        labels, _ = g.run(10)
+        assert np.array_equal(labels, final)
+    rows = [x.split() for x in open(cache).read().split("\n") if x.strip()]
+    streams = [r for r in rows if r[2] == "12"]
+    assert len(streams) == 1 and streams[0][5] in ("1", "3")
+    # force the other choice through the cache: the labels do not change
+    other = "3" if streams[0][5] == "1" else "1"
+    with open(cache, "w") as f:
+        f.write("\n".join(" ".join(r[:5] + [other]) if r[2] == "12" else " ".join(r) for r in rows) + "\n")
+    with _abi.Graph(ctx, off, nbr, method=method) as g:
+        g.autotune(reps=1, cache_path=cache)
+        assert g.tuned_streams == int(other)
+        labels, _ = g.run(10)
+        assert np.array_equal(labels, final)
