@@ -10,13 +10,15 @@
 // keys per thread, radix or bitonic, grids).  The choice never changes
 // labels (every configuration computes the same rule), which is how it is
 // tested.  Last, where an iteration may run its class kernels on three
-// streams, whole iterations are timed on one stream and on three and the
-// faster is kept (the overlap helps while the label array stays in the
-// Infinity Cache and can hurt once it does not).  Choices can be cached in a
+// streams, whole iterations are timed on one stream and on three; the
+// choice by size stays unless the other is more than 3 % faster (the
+// overlap helps while the label array stays in the Infinity Cache and can
+// hurt once it does not).  Choices can be cached in a
 // text file, one line per class (class 12 = the stream choice, 1 or 3):
 //   <arch> <method> <class> <log2 vertices> <log2 edges> <option>
 // and are reused for a graph whose class has the same arch, method and sizes.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -134,18 +136,28 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
             if (c.arch == arch && c.method == g->opts.method && c.cls == kStreamsLine && c.lv == lv && c.le == le)
                 mode = c.opt;
         if (!mode) {
-            float t[2] = {0.f, 0.f};
-            for (int q = 0; q < 2 && rc == GALP_OK; ++q) {
-                g->streams = (int8_t)(q ? 3 : 1);
-                rc = one_iteration(g);   // warm-up
-                if (rc != GALP_OK) break;
-                (void)hipEventRecord(e0, g->stream);
-                for (int k = 0; k < reps && rc == GALP_OK; ++k) rc = one_iteration(g);
-                (void)hipEventRecord(e1, g->stream);
-                if (hipEventSynchronize(e1) != hipSuccess) rc = fail(g->ctx, GALP_EHIP, "autotune timing");
-                if (rc == GALP_OK) (void)hipEventElapsedTime(&t[q], e0, e1);
-            }
-            mode = t[1] < t[0] ? 3 : 1;
+            // the two choices timed alternately, best of three rounds each;
+            // the size rule's choice stays unless the other is faster by
+            // more than 3 % (on labels[i] = i the two are often within noise,
+            // while later iterations favour the size rule)
+            g->streams = 0;
+            const int def = concurrent_classes(g) ? 3 : 1;
+            float best[2] = {1e30f, 1e30f};
+            for (int round = 0; round < 3 && rc == GALP_OK; ++round)
+                for (int q = 0; q < 2 && rc == GALP_OK; ++q) {
+                    g->streams = (int8_t)(q ? 3 : 1);
+                    rc = one_iteration(g);   // warm-up
+                    if (rc != GALP_OK) break;
+                    (void)hipEventRecord(e0, g->stream);
+                    for (int k = 0; k < reps && rc == GALP_OK; ++k) rc = one_iteration(g);
+                    (void)hipEventRecord(e1, g->stream);
+                    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(g->ctx, GALP_EHIP, "autotune timing");
+                    float ms = 0.f;
+                    if (rc == GALP_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+                    best[q] = std::min(best[q], ms);
+                }
+            const float td = best[def == 3], to = best[def != 3];
+            mode = to < 0.97f * td ? (def == 3 ? 1 : 3) : def;
             if (rc == GALP_OK) fresh.push_back(CacheLine{arch, g->opts.method, kStreamsLine, lv, le, mode});
         }
         g->streams = (int8_t)mode;
