@@ -87,6 +87,10 @@ def train_network(ctx, off, nbr, niter):
         g.log_enable((niter + 1) * max(big, 1))
         g.run(niter, want_labels=False)
         x, y = g.log_read()
+    # the device appends samples in no fixed order: a canonical order first,
+    # so that the seeded subsample and training give the same network every run
+    order = np.lexsort((y, x[:, 3], x[:, 2], x[:, 1], x[:, 0]))
+    x, y = x[order], y[order]
     if len(x) > 3_000_000:   # a seeded subsample keeps training time flat on large graphs
         idx = np.random.default_rng(0).choice(len(x), 3_000_000, replace=False)
         x, y = x[idx], y[idx]
