@@ -31,6 +31,11 @@ value        = GTEPS of the first variant (hash_learned_tuned by default)
              = m * iterations / wall time of the timed
                steps (host clock around the steps, device synchronised, no
                instrumentation)
+gather_ceiling = k_gather_probe on the same graph and layout: neighbour-id reads
+               and label gathers only (8 B per edge), the best rate an LP kernel
+               can reach on this access pattern; dominant_kernel_frac and
+               iteration_frac compare the roofline kernel and the whole
+               iteration (algorithmic bytes per second) with it
 roofline     = the dominant kernel of the first variant: its algorithmic bytes per
                launch / its average launch time, from HIP events on the launch
                stream over a second, instrumented pass of the same steps,
@@ -321,6 +326,16 @@ def main():
                 log(f"    {k['name']:18s} {k['launches']:5d} launches {k['total_ms'] / k['launches'] * 1e3:9.1f} us avg "
                     f"{k['alg_bytes'] / (k['total_ms'] * 1e-3) / 1e9 if k['total_ms'] > 0 else 0:8.1f} GB/s alg")
 
+    # the gather ceiling: one pass of neighbour-id reads and label gathers
+    # only (k_gather_probe), on the layout every in-core variant uses
+    probe_ms = None
+    if not any("_ooc_" in nm for nm in names):
+        with _abi.Graph(ctx, off, nbr) as g:
+            g.init()
+            probe_ms, _ = g.gather_probe(reps=5)
+        log(f"bench: gather probe: {probe_ms * 1e3:.1f} us per pass of {m} gathers "
+            f"-> {m / (probe_ms * 1e-3) / 1e9:.1f} G gathers/s")
+
     head = names[0]
     main_v = variants[head]
     wall = main_v["wall_s"]
@@ -364,6 +379,12 @@ def main():
                      "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "iteration_roofline": {"alg_bytes": b_iter, "achieved_GBps": round(b_iter / per_iter_s / 1e9, 1),
                                "frac": round(b_iter / per_iter_s / 1e9 / HBM_PEAK_GBS, 4)},
+        **({"gather_ceiling": {
+            "kernel": "k_gather_probe", "ms_per_pass": round(probe_ms, 4),
+            "gathers_per_s": round(m / (probe_ms * 1e-3), 1), "alg_GBps": round(8.0 * m / (probe_ms * 1e-3) / 1e9, 1),
+            "dominant_kernel_frac": round(achieved / (8.0 * m / (probe_ms * 1e-3) / 1e9), 4),
+            "iteration_frac": round((b_iter / per_iter_s) / (8.0 * m / (probe_ms * 1e-3)), 4)}}
+           if probe_ms else {}),
         "variants": vg,
         **({"ooc_link_GBps": link, "ooc_buffer_pow": args.buffer_pow} if link else {}),
         "device": {"name": dev["name"], "arch": dev["arch"], "cus": dev["compute_units"]},

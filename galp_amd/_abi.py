@@ -106,6 +106,7 @@ def lib():
         "galp_lp_set_labels": (ctypes.c_int, [vp, vp]),
         "galp_profile_read": (ctypes.c_int, [vp, ctypes.POINTER(KernelStat), i32, ctypes.POINTER(i32)]),
         "galp_profile_reset": (ctypes.c_int, [vp]),
+        "galp_graph_gather_probe": (ctypes.c_int, [vp, i32, ctypes.POINTER(dbl), ctypes.POINTER(ctypes.c_uint64)]),
         "galp_last_error": (ctypes.c_char_p, [vp]),
         "galp_load_graph": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(HostCsr))]),
         "galp_save_bin": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(HostCsr)]),
@@ -338,6 +339,13 @@ class Graph:
 
     def profile_reset(self):
         self.ctx.check(lib().galp_profile_reset(self.h))
+
+    def gather_probe(self, reps=5):
+        """(ms per pass, checksum) of the gather-only probe over this graph."""
+        ms = ctypes.c_double()
+        cs = ctypes.c_uint64()
+        self.ctx.check(lib().galp_graph_gather_probe(self.h, reps, ctypes.byref(ms), ctypes.byref(cs)))
+        return ms.value, cs.value
 
 
     # ---- autotuner ---------------------------------------------------------

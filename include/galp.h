@@ -186,6 +186,16 @@ GALP_API int galp_profile_read(galp_graph *g, galp_kernel_stat *out,
                                int32_t max, int32_t *count);
 GALP_API int galp_profile_reset(galp_graph *g);
 
+/* Diagnostic, no reference counterpart: the gather ceiling of this graph's
+ * access pattern.  One pass reads every neighbour id (sequentially,
+ * non-temporal) and gathers that neighbour's current label, as every LP
+ * kernel does, with nothing else: no counting, no label writes, only a
+ * 64-bit sum of the gathered labels (*checksum, may be NULL).  Runs one pass
+ * for the checksum, then `reps` timed ones; *ms_per_pass gets their mean
+ * (HIP events on the graph's stream).  In core only. */
+GALP_API int galp_graph_gather_probe(galp_graph *g, int32_t reps, double *ms_per_pass,
+                                     uint64_t *checksum);
+
 /* ---- learned dispatch: the distinct-label predictor ----------------------
  * A 4 -> H -> H -> 1 MLP (ReLU, ReLU, linear; fp32; H in {8, 16, 32}) with
  * parameters laid out as W1[H][4], b1[H], W2[H][H], b2[H], W3[H], b3.

@@ -283,3 +283,19 @@ def test_rmat20_concurrent_classes(ctx, method):
         for _ in range(3):   # capture, then replays
             labels, _ = g.run(10)
             assert np.array_equal(labels, final)
+
+
+def test_gather_probe_checksum(ctx):
+    """The gather probe reads every neighbour's current label: with the input
+    numbering kept, its checksum is the sum of labels[nbr] (labels[i] = i
+    after init, then the labels of a few iterations)."""
+    off, nbr = _abi.gen_rmat(16, 16, seed=31)
+    with _abi.Graph(ctx, off, nbr, reorder=0) as g:
+        g.init()
+        ms, cs = g.gather_probe(reps=2)
+        assert ms > 0 and cs == int(nbr.astype(np.uint64).sum())
+        for _ in range(3):
+            g.step()
+        lab = g.labels()
+        _, cs = g.gather_probe(reps=1)
+        assert cs == int(lab[nbr].astype(np.uint64).sum())
