@@ -1,20 +1,24 @@
 // autotune.hip -- picks each degree class's kernel configuration by timing.
 //
 // No reference counterpart (the reference fixes its block sizes at compile
-// time, lfht.cuh:127-165).  For every degree class with vertices, hubs
-// included, each of its configurations is launched `reps` times on the
-// graph's current labels and timed with HIP events; the fastest is kept.
-// The configurations are the method's: launch_class in lp_hash.hip
-// (workgroup size x slots per lane and blocks per CU; for hubs the tile and
-// merge workgroup sizes) and sort_class in lp_sort.hip (workgroup size x
-// keys per thread, radix or bitonic, grids).  The choice never changes
-// labels (every configuration computes the same rule), which is how it is
-// tested.  Last, where an iteration may run its class kernels on three
-// streams, whole iterations are timed on one stream and on three; the
-// choice by size stays unless the other is more than 3 % faster (the
-// overlap helps while the label array stays in the Infinity Cache and can
-// hurt once it does not).  Choices can be cached in a
-// text file, one line per class (class 12 = the stream choice, 1 or 3):
+// time, lfht.cuh:127-165).  Each round sets every untuned class to option 0,
+// then 1, then 2, and times a real run of kTuneIters iterations from
+// labels[i] = i under the graph's own dispatch (learned when it has a
+// network), reading every class's device time from per-kernel events; each
+// class keeps its fastest option over the rounds.  Timing whole runs, not
+// single launches on the first iteration's labels, weighs the converged
+// iterations, where rows change path and most of a run's time goes.  The
+// configurations are the method's: launch_class in lp_hash.hip (workgroup
+// size x slots per lane and blocks per CU; for hubs the tile and merge
+// workgroup sizes) and sort_class in lp_sort.hip (workgroup size x keys per
+// thread, the sort algorithm, grids).  The choice never changes labels (every
+// configuration computes the same rule), which is how it is tested.  Last,
+// where an iteration may run its class kernels on three streams, whole runs
+// are timed on one stream and on three; the choice by size stays unless the
+// other is more than 3 % faster (the overlap helps while the label array
+// stays in the Infinity Cache and can hurt once it does not).  Choices can
+// be cached in a text file, one line per class (class 12 = the stream
+// choice, 1 or 3):
 //   <arch> <method> <class> <log2 vertices> <log2 edges> <option>
 // and are reused for a graph whose class has the same arch, method and sizes.
 
@@ -33,10 +37,39 @@ namespace galp {
 constexpr int kTuneOptions = 3;
 constexpr int kStreamsLine = NUM_BUCKETS;   // the cache line of the stream choice
 
-// The LP kernels of one in-core iteration (no bookkeeping kernel).
-static int one_iteration(galp_graph *g) {
-    return g->opts.method == GALP_METHOD_HASH ? launch_hash_iteration_incore(g)
-                                              : launch_sort_iteration(g, full_view(g));
+constexpr int kTuneIters = 10;   // iterations of each timed run
+
+// Per-class device time (ms) of a run of `niter` iterations from labels[i]
+// = i under the current options, from per-kernel events (which also keep the
+// run on one stream).  The caller's profile totals are kept aside and put
+// back.
+static int run_class_times(galp_graph *g, int niter, double *t) {
+    const std::vector<KernelAgg> saved = g->prof.aggs;
+    for (KernelAgg &a : g->prof.aggs) {
+        a.launches = 0;
+        a.total_ms = 0.0;
+        a.alg_bytes = 0.0;
+    }
+    const bool on = g->prof.on;
+    g->prof.on = true;
+    galp_run_result res{};
+    const int rc = galp_lp_run(g, niter, 0, nullptr, &res);
+    g->prof.on = on;
+    for (auto &pl : g->prof.pending) {   // a failed run may leave some
+        if (pl.e0) g->prof.pool.push_back(pl.e0);
+        if (pl.e1) g->prof.pool.push_back(pl.e1);
+    }
+    g->prof.pending.clear();
+    for (int b = 0; b < NUM_BUCKETS; ++b) t[b] = 0.0;
+    for (const KernelAgg &a : g->prof.aggs) {
+        int b = -1;
+        for (int c = 0; c < NUM_BUCKETS; ++c)
+            if (a.name == kBucketName[c]) b = c;
+        if (a.name == "hub_tile" || a.name == "hub_merge" || a.name == "hub_final" || a.name == "hub_sort") b = B_HUB;
+        if (b >= 0) t[b] += a.total_ms;
+    }
+    g->prof.aggs = saved;   // no launch is pending after the run
+    return rc;
 }
 
 static int ilog2(int64_t x) { return x > 0 ? 63 - __builtin_clzll((unsigned long long)x) : -1; }
@@ -82,22 +115,19 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
     const bool hash = g->opts.method == GALP_METHOD_HASH;
     if ((hash && g->opts.variant != 1) || g->opts.policy != GALP_POLICY_INCORE || g->ooc)
         return fail(g->ctx, GALP_EUNSUPPORTED, "the autotuner tunes policy 0 (in-core), and method 1 with -l 1");
-    {   // tune on labels[i] = i, the first iteration's (and heaviest) workload
-        const int rc0 = galp_lp_init(g);
-        if (rc0 != GALP_OK) return rc0;
-    }
     const std::string arch = g->ctx->prop.gcnArchName;
     std::vector<CacheLine> cache = read_cache(cache_path);
     std::vector<CacheLine> fresh;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-        return fail(g->ctx, GALP_EHIP, "autotune events");
-    int rc = GALP_OK;
-    const bool prof = g->prof.on;
+    // the runs below use the graph's own dispatch (learned needs its network)
     const int32_t dispatch = g->opts.dispatch;
-    g->prof.on = false;
-    g->opts.dispatch = GALP_DISPATCH_PLAIN;   // time each class over all of its vertices
-    for (int b = 0; b <= B_HUB && rc == GALP_OK; ++b) {
+    if (dispatch == GALP_DISPATCH_LEARNED && !(g->learn && g->learn->H)) g->opts.dispatch = GALP_DISPATCH_PLAIN;
+    const int32_t capture = g->opts.graph_capture;
+    g->opts.graph_capture = 0;
+    invalidate_captured(g);
+    int rc = GALP_OK;
+    bool need[NUM_BUCKETS] = {};
+    bool any = false;
+    for (int b = 0; b <= B_HUB; ++b) {
         const BucketRange &r = g->bucket[b];
         if (!r.nv) continue;
         const int lv = ilog2(r.nv), le = ilog2(r.edges);
@@ -107,68 +137,65 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
                 g->tune[b] = (int8_t)c.opt;
                 hit = true;
             }
-        if (hit) continue;
-        float best = 1e30f;
-        int bo = 0;
-        for (int o = 0; o < kTuneOptions && rc == GALP_OK; ++o) {
-            g->tune[b] = (int8_t)o;
-            rc = hash ? launch_hash_class(g, b) : launch_sort_class(g, b);   // warm-up launch
-            if (rc != GALP_OK) break;
-            (void)hipEventRecord(e0, g->stream);
-            for (int k = 0; k < reps && rc == GALP_OK; ++k)
-                rc = hash ? launch_hash_class(g, b) : launch_sort_class(g, b);
-            (void)hipEventRecord(e1, g->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(g->ctx, GALP_EHIP, "autotune timing");
-            float ms = 0.f;
-            if (rc == GALP_OK) (void)hipEventElapsedTime(&ms, e0, e1);
-            if (rc == GALP_OK && ms < best) {
-                best = ms;
-                bo = o;
-            }
-        }
-        g->tune[b] = (int8_t)bo;
-        fresh.push_back(CacheLine{arch, g->opts.method, b, lv, le, bo});
+        need[b] = !hit;
+        any = any || need[b];
     }
-    if (rc == GALP_OK && concurrent_allowed(g)) {   // whole iterations: one stream or three
+    if (any) {   // every option of every class timed together, `reps` rounds
+        double best[NUM_BUCKETS];
+        int8_t bo[NUM_BUCKETS];
+        for (int b = 0; b < NUM_BUCKETS; ++b) {
+            best[b] = 1e300;
+            bo[b] = g->tune[b];
+        }
+        for (int round = 0; round < reps && rc == GALP_OK; ++round)
+            for (int o = 0; o < kTuneOptions && rc == GALP_OK; ++o) {
+                for (int b = 0; b < NUM_BUCKETS; ++b)
+                    if (need[b]) g->tune[b] = (int8_t)o;
+                double t[NUM_BUCKETS];
+                rc = run_class_times(g, kTuneIters, t);
+                for (int b = 0; b < NUM_BUCKETS && rc == GALP_OK; ++b)
+                    if (need[b] && t[b] < best[b]) {
+                        best[b] = t[b];
+                        bo[b] = (int8_t)o;
+                    }
+            }
+        for (int b = 0; b < NUM_BUCKETS; ++b)
+            if (need[b]) {
+                g->tune[b] = bo[b];
+                const BucketRange &r = g->bucket[b];
+                if (rc == GALP_OK) fresh.push_back(CacheLine{arch, g->opts.method, b, ilog2(r.nv), ilog2(r.edges), bo[b]});
+            }
+    }
+    if (rc == GALP_OK && concurrent_allowed(g)) {   // whole runs: one stream or three
         const int lv = ilog2(g->n), le = ilog2(g->m);
         int mode = 0;
         for (const CacheLine &c : cache)
             if (c.arch == arch && c.method == g->opts.method && c.cls == kStreamsLine && c.lv == lv && c.le == le)
                 mode = c.opt;
         if (!mode) {
-            // the two choices timed alternately, best of three rounds each;
-            // the size rule's choice stays unless the other is faster by
-            // more than 3 % (on labels[i] = i the two are often within noise,
-            // while later iterations favour the size rule)
+            // the two choices alternately, best of `reps` rounds each; the
+            // choice by size stays unless the other is faster by more than 3 %
             g->streams = 0;
             const int def = concurrent_classes(g) ? 3 : 1;
-            float best[2] = {1e30f, 1e30f};
-            for (int round = 0; round < 3 && rc == GALP_OK; ++round)
+            double tbest[2] = {1e300, 1e300};
+            for (int round = 0; round < reps && rc == GALP_OK; ++round)
                 for (int q = 0; q < 2 && rc == GALP_OK; ++q) {
                     g->streams = (int8_t)(q ? 3 : 1);
-                    rc = one_iteration(g);   // warm-up
-                    if (rc != GALP_OK) break;
-                    (void)hipEventRecord(e0, g->stream);
-                    for (int k = 0; k < reps && rc == GALP_OK; ++k) rc = one_iteration(g);
-                    (void)hipEventRecord(e1, g->stream);
-                    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(g->ctx, GALP_EHIP, "autotune timing");
-                    float ms = 0.f;
-                    if (rc == GALP_OK) (void)hipEventElapsedTime(&ms, e0, e1);
-                    best[q] = std::min(best[q], ms);
+                    galp_run_result res{};
+                    rc = galp_lp_run(g, kTuneIters, 0, nullptr, &res);
+                    if (rc == GALP_OK) tbest[q] = std::min(tbest[q], res.device_ms);
                 }
-            const float td = best[def == 3], to = best[def != 3];
-            mode = to < 0.97f * td ? (def == 3 ? 1 : 3) : def;
+            const double td = tbest[def == 3], to = tbest[def != 3];
+            mode = to < 0.97 * td ? (def == 3 ? 1 : 3) : def;
             if (rc == GALP_OK) fresh.push_back(CacheLine{arch, g->opts.method, kStreamsLine, lv, le, mode});
         }
         g->streams = (int8_t)mode;
     }
-    g->prof.on = prof;
     g->opts.dispatch = dispatch;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    g->opts.graph_capture = capture;
     invalidate_captured(g);
     if (rc != GALP_OK) return rc;
-    // the timed launches wrote next labels and the change counter: start over
+    // the timed runs moved the labels on: start over
     rc = galp_lp_init(g);
     if (rc != GALP_OK) return rc;
     if (cache_path && !fresh.empty()) {

@@ -244,11 +244,14 @@ GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max,
  * Each degree class of either method has kTuneOptions = 3 kernel
  * configurations (0 = default; the others change the workgroup size, slots
  * or keys per lane, the sort algorithm or blocks per CU).
- * galp_graph_autotune times every option of every class with vertices on
- * labels[i] = i (`reps` launches each) and keeps the fastest; then, in core,
- * it times whole iterations with the class kernels on one stream and on
- * three and keeps the faster.  It leaves the graph initialised
- * (labels[i] = i).  cache_path (may be NULL) names a text file of earlier
+ * galp_graph_autotune runs 10 iterations from labels[i] = i under each
+ * option (all untuned classes at once, the graph's own dispatch), `reps`
+ * rounds, reads each class's device time from per-kernel events and keeps
+ * its fastest option; then, in core, it times such runs with the class
+ * kernels on one stream and on three and keeps the size rule's choice
+ * unless the other is more than 3 % faster.  It leaves the graph
+ * initialised (labels[i] = i) and the per-kernel profile totals as they
+ * were.  cache_path (may be NULL) names a text file of earlier
  * choices keyed by device arch, method and the class's log2 sizes; hits are
  * reused without timing and new choices are appended.  chosen (may be NULL)
  * receives 16 entries: the 12 class options, then the stream choice (1 or 3;
