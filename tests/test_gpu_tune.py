@@ -118,3 +118,19 @@ def test_autotune_times_streams(ctx, tmp_path, method):
         assert g.tuned_streams == int(other)
         labels, _ = g.run(10)
         assert np.array_equal(labels, final)
+
+
+def test_autotune_keeps_profile_totals(ctx):
+    """The autotuner times classes with per-kernel events but leaves the
+    caller's profile totals as they were."""
+    off, nbr = _abi.gen_rmat(15, 16, seed=26)
+    final, _, _ = oracle.lp_run(off, nbr, niter=10)
+    with _abi.Graph(ctx, off, nbr, profile=1) as g:
+        g.run(10, want_labels=False)
+        before = {k["name"]: (k["launches"], k["total_ms"]) for k in g.profile()}
+        assert sum(v[0] for v in before.values()) > 0
+        g.autotune(reps=1)
+        after = {k["name"]: (k["launches"], k["total_ms"]) for k in g.profile()}
+        assert after == before
+        labels, _ = g.run(10)
+        assert np.array_equal(labels, final)
