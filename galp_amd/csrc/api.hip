@@ -485,7 +485,8 @@ bool concurrent_allowed(const galp_graph *g) {
 
 bool concurrent_classes(const galp_graph *g) {
     if (!concurrent_allowed(g) || g->streams == 1) return false;
-    return g->streams == 3 || (g->m >= kConcurrentMinEdges && g->n <= kConcurrentMaxVertices);
+    const int64_t span = (g->batch && !g->reorder) ? g->batch->max_n : g->n;   // ids a gather can reach
+    return g->streams == 3 || (g->m >= kConcurrentMinEdges && span <= kConcurrentMaxVertices);
 }
 
 int side_begin(galp_graph *g, int k, hipStream_t *main_stream) {

@@ -173,7 +173,10 @@ GALP_API int galp_graph_set_batch(galp_graph *g, int64_t ngraphs, const int64_t 
     g->batch = B;
     B->ngraphs = ngraphs;
     std::vector<int64_t> gn((size_t)ngraphs);
-    for (int64_t k = 0; k < ngraphs; ++k) gn[(size_t)k] = vbase[k + 1] - vbase[k];
+    for (int64_t k = 0; k < ngraphs; ++k) {
+        gn[(size_t)k] = vbase[k + 1] - vbase[k];
+        B->max_n = std::max(B->max_n, gn[(size_t)k]);
+    }
     const size_t n = (size_t)std::max<int64_t>(g->n, 1), ng = (size_t)ngraphs;
     if (hipMalloc((void **)&B->d_gid, 4 * n) != hipSuccess || hipMalloc((void **)&B->d_gn, 8 * ng) != hipSuccess ||
         hipMalloc((void **)&B->d_gcount, 8 * ng) != hipSuccess || hipMalloc((void **)&B->d_gstop, 4 * ng) != hipSuccess ||

@@ -201,6 +201,7 @@ namespace galp {
 // graph vertex count, change counter, stop flag and iterations done.
 struct BatchState {
     int64_t ngraphs = 0;
+    int64_t max_n = 0;   // vertices of the largest graph
     int32_t *d_gid = nullptr;
     int64_t *d_gn = nullptr, *d_vbase = nullptr;
     unsigned long long *d_gcount = nullptr;
@@ -298,10 +299,12 @@ int launch_iter_end(galp_graph *g);
 // the -l 0 and li layouts; otherwise as the autotuner chose (g->streams), or
 // by size: from kConcurrentMinEdges edges (below, an iteration's kernels are
 // short and the fork and join cost more than the overlap gains: a
-// 20000-vertex graph ran at half speed with them) while the label array
-// fits in half of the 256 MB Infinity Cache (kConcurrentMaxVertices: at
-// RMAT-26, 268 MB of labels, overlapping classes ran 3-6 % slower, at
-// RMAT-22 and LFR-8M 2-7 % faster).
+// 20000-vertex graph ran at half speed with them) while the labels one
+// gather can reach fit in half of the 256 MB Infinity Cache
+// (kConcurrentMaxVertices: at RMAT-26, 268 MB of labels, overlapping
+// classes ran 3-6 % slower, at RMAT-22 and LFR-8M 2-7 % faster).  In a
+// batch kept in its input numbering a gather stays inside its own graph, so
+// the largest graph counts, not the batch.
 constexpr int64_t kConcurrentMinEdges = (int64_t)1 << 24;
 constexpr int64_t kConcurrentMaxVertices = (int64_t)1 << 25;
 bool concurrent_allowed(const galp_graph *g);
