@@ -333,3 +333,16 @@ def test_learned_rmat20_concurrent_classes(ctx, net):
         assert routed > 0
     else:
         run_learned(ctx, off, nbr, 6, "rmat20-rand", _abi.mlp_init(32, 7), 32, 3.0)
+
+
+def test_learned_lfr_1m_per_iteration(ctx):
+    """Learned dispatch on an LFR-style graph of 1M vertices with a network
+    trained on the GPU from the graph's own logged run: every iteration
+    bit-exact against the oracle."""
+    off, nbr = _abi.gen_lfr(1_000_000, 2.5, 20.0, 2000, 1.5, 50, 50000, 0.3, seed=10)
+    with _abi.Graph(ctx, off, nbr) as g:
+        g.log_enable(11 * int((np.diff(off) > 64).sum()))
+        g.run(10, want_labels=False)
+        x, y = g.log_read()
+    p, _ = _abi.mlp_train(ctx, 32, _abi.mlp_init(32, 1), x, y, epochs=2, batch=512, lr=3e-3, seed=2)
+    run_learned(ctx, off, nbr, 10, "lfr1m-trained", p, 32, 14.0)

@@ -299,3 +299,13 @@ def test_gather_probe_checksum(ctx):
         lab = g.labels()
         _, cs = g.gather_probe(reps=1)
         assert cs == int(lab[nbr].astype(np.uint64).sum())
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_lfr_1m_per_iteration(ctx, method):
+    """An LFR-style graph (1M vertices, power-law degrees up to 2000 and
+    communities of 50..50000, mixing 0.3): every iteration bit-exact against
+    the oracle.  Most of its edges sit in rows of 65..2000, the LDS and
+    register-sort classes, with many distinct labels per row."""
+    off, nbr = _abi.gen_lfr(1_000_000, 2.5, 20.0, 2000, 1.5, 50, 50000, 0.3, seed=9)
+    check_per_iteration(ctx, off, nbr, 10, "lfr1m", method=method)
