@@ -346,3 +346,18 @@ def test_learned_lfr_1m_per_iteration(ctx):
         x, y = g.log_read()
     p, _ = _abi.mlp_train(ctx, 32, _abi.mlp_init(32, 1), x, y, epochs=2, batch=512, lr=3e-3, seed=2)
     run_learned(ctx, off, nbr, 10, "lfr1m-trained", p, 32, 14.0)
+
+
+def test_learned_rmat24_trained_per_iteration(ctx):
+    """Learned dispatch at RMAT scale 24 (m = 521M, hub rows of hundreds of
+    candidate tiles merged by k_cand_merge) with a network trained on the
+    GPU from the graph's own logged run: every iteration bit-exact against
+    the oracle, and some multi-tile rows take the candidate path."""
+    off, nbr = _abi.gen_rmat(24, 16, seed=24)
+    with _abi.Graph(ctx, off, nbr) as g:
+        g.log_enable(11 * int((np.diff(off) > 64).sum()))
+        g.run(10, want_labels=False)
+        x, y = g.log_read()
+    p, _ = _abi.mlp_train(ctx, 32, _abi.mlp_init(32, 1), x, y, epochs=2, batch=512, lr=3e-3, seed=2)
+    routed, _ = run_learned(ctx, off, nbr, 10, "rmat24-trained", p, 32, 14.0)
+    assert routed > 0
