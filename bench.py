@@ -1,12 +1,14 @@
 #!/usr/bin/env python3
 """GTEPS per label-propagation iteration on one MI355X.
 
-Workload: RMAT scale 22, edge factor 16 (a=.57, b=.19, c=.19, seed 22),
-symmetrised, deduplicated, self-loops dropped, ids permuted; int32 CSR with
-m = 128,302,318 directed edges (--scale for other scales; --graph lfr for the
-LFR-style graph of 8M vertices).  One step = one full LP run of 10
-synchronous iterations from labels[i] = i (the reference's timed main loop,
-lfht.cuh:87-106), inputs resident in HBM.  Prints one JSON line.
+Workload: RMAT scale 26, edge factor 16 (a=.57, b=.19, c=.19, seed 26),
+symmetrised, deduplicated, self-loops dropped, ids permuted; int32
+neighbour ids, int64 offsets, m = 2,103,849,724 directed edges -- the scale
+the learned-vs-plain and %-of-HBM targets are quoted on (--scale 22 for the
+RMAT-22 configuration; --graph lfr for the LFR-style graph of 8M
+vertices).  One step = one full LP run of 10 synchronous iterations from
+labels[i] = i (the reference's timed main loop, lfht.cuh:87-106), inputs
+resident in HBM.  Prints one JSON line.
 
 Variants, each timed over the same steps:
   hash_learned  method 1, learned dispatch: the 4-32-32-1 network is trained
@@ -27,6 +29,8 @@ Variants, each timed over the same steps:
   batched       512 planted-partition graphs (n up to 20000; --batch-graphs,
                 --batch-nmax) in one batched run (separate JSON object, not
                 the value)
+Default variants: hash_learned_tuned (the value) and hash_tuned, the pair
+learned_vs_plain compares.
 value        = GTEPS of the first variant (hash_learned_tuned by default)
              = m * iterations / wall time of the timed
                steps (host clock around the steps, device synchronised, no
@@ -41,12 +45,22 @@ roofline     = the dominant kernel of the first variant: its algorithmic bytes p
                stream over a second, instrumented pass of the same steps,
                against 8 TB/s; traffic = the HBM bytes per launch of the same
                kernel, workload and variant measured with rocprofv3 counters
-               (profiles/hbm_traffic.json), null where none was measured
-cpu_baseline = the CPU oracle (oracle/lp_oracle.c, OpenMP) on the same graph
-               and iterations; its final labels are the parity check of every
-               variant
+               and recorded in profiles/hbm_traffic.json together with a hash
+               of the kernel sources they were measured on: null (and
+               traffic_stale true) when no entry matches or the kernels have
+               changed since
+cpu_baseline = the std::map-structured CPU baseline (oracle/lp_baseline.cpp,
+               label_propagator.h:52-66 made synchronous, OpenMP
+               schedule(dynamic,1)) on a bounded sample of the workload: one
+               iteration from the labels of the run's 5th iteration over every
+               k-th vertex (k sized for ~10 s), on all OpenMP threads (value)
+               and on one; its labels are checked against the oracle's.  The
+               full run of the dense CPU oracle (oracle/lp_oracle.c) gives
+               the final labels every variant must equal (labels_bit_exact)
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import sys
@@ -66,19 +80,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload, variant, kernel):
+def kernel_source_hash():
+    """sha256 (first 16 hex digits) of the device sources, galp_amd/csrc/*.hip
+    and *.hpp in name order: the key that ties a traffic measurement to the
+    kernels it was taken on."""
+    h = hashlib.sha256()
+    for path in sorted(glob.glob(os.path.join(ROOT, "galp_amd", "csrc", "*.hip")) +
+                       glob.glob(os.path.join(ROOT, "galp_amd", "csrc", "*.hpp"))):
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(workload, variant, kernel, source_hash=None):
     """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE counters,
     each scaled by what it reads for a copy of known size), measured for this
-    workload and variant; None when the measurement file holds no such entry."""
+    workload and variant on kernel sources whose hash is source_hash (the
+    current sources when None).  Returns (bytes or None, stale): stale is True
+    when an entry exists but was measured on other kernel sources, and the
+    figure is then withheld."""
     try:
         with open(TRAFFIC_FILE) as f:
             entries = json.load(f).get("entries", [])
     except (OSError, ValueError):
-        return None
+        return None, False
+    want = source_hash or kernel_source_hash()
+    stale = False
     for e in entries:
         if e.get("workload") == workload and e.get("variant") == variant and e.get("kernel") == kernel:
-            return float(e["bytes_per_launch"])
-    return None
+            if e.get("source_hash") == want:
+                return float(e["bytes_per_launch"]), False
+            stale = True
+    return None, stale
 
 
 def train_network(ctx, off, nbr, niter):
@@ -210,6 +244,67 @@ def time_variant(ctx, off, nbr, steps, warmup, niter, mlp=None, tune=False, **op
     return out
 
 
+def cpu_legs(off, nbr, gname, niter, variants):
+    """The dense oracle's full run (final labels: labels_bit_exact of every
+    variant) and the std::map-structured CPU baseline on a bounded sample."""
+    import numpy as np
+    import oracle
+    n, m = len(off) - 1, len(nbr)
+    t = time.perf_counter()
+    prev = np.arange(n, dtype=np.int32)
+    mid = None
+    k = 0
+    for it in range(niter):
+        nxt, c = oracle.lp_iterate(off, nbr, prev, nlabels=n)
+        prev = nxt
+        k += 1
+        if it + 1 == min(5, niter):
+            mid = prev.copy()
+        if c <= n * 1e-5:
+            break
+    log(f"bench: CPU oracle: {k} iterations in {time.perf_counter() - t:.1f} s")
+    res = {"labels_bit_exact": {name: bool(np.array_equal(v["labels"], prev)) for name, v in variants.items()}}
+    if mid is None:
+        mid = prev
+    deg = np.diff(off)
+    nthreads = oracle.baseline_threads()
+
+    def timed(target_s, threads):
+        # every k-th vertex: first a small sample for the rate, then one sized
+        # for target_s seconds
+        for target in (1e6, None):
+            if target is None:
+                rate = edges / max(secs, 1e-6)
+                target = rate * target_s
+            stride = max(1, int(round(m / max(target, 1.0))))
+            vl = np.arange(0, n, stride, dtype=np.int64)
+            edges = int(deg[vl].sum())
+            t0 = time.perf_counter()
+            got, _ = oracle.map_iterate(off, nbr, mid, vlist=vl, nthreads=threads)
+            secs = time.perf_counter() - t0
+        return edges, secs, stride, got, vl
+
+    e_all, s_all, st_all, got, vl = timed(10.0, nthreads)
+    e_one, s_one, st_one, got1, vl1 = timed(10.0, 1)
+    # parity of the baseline itself on its sample: the oracle's next labels
+    want, _ = oracle.lp_iterate(off, nbr, mid, nlabels=n)
+    ok = bool(np.array_equal(got[vl], want[vl]) and np.array_equal(got1[vl1], want[vl1]))
+    it_mid = min(5, niter) + 1
+    res["cpu_baseline"] = {
+        "value": round(e_all / s_all / 1e9, 5), "unit": "GTEPS", "cores": nthreads, "kind": "port",
+        "sample": f"{gname}, iteration {it_mid} (from the labels after iteration {it_mid - 1}) over every "
+                  f"{st_all}-th vertex ({e_all} edges, {s_all:.2f} s) on {nthreads} OpenMP threads, "
+                  f"std::map histogram per vertex, schedule(dynamic,1)",
+        "single_thread": {"value": round(e_one / s_one / 1e9, 5), "unit": "GTEPS", "cores": 1,
+                          "sample": f"every {st_one}-th vertex ({e_one} edges, {s_one:.2f} s)"},
+        "labels_equal_oracle": ok,
+        "cpu_model": oracle.cpu_model(), "nproc": os.cpu_count(),
+        "threads_available": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+    }
+    log(f"bench: CPU baseline: {res['cpu_baseline']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,10 +313,12 @@ def main():
     ap.add_argument("--graph", choices=("rmat", "lfr"), default="rmat",
                     help="rmat: RMAT --scale; lfr: LFR-style graph of --lfr-n vertices (gamma 2.5, mean degree 20, "
                          "max degree 2000, community sizes 50..50000 with exponent 1.5, mu 0.3, seed 8)")
-    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--lfr-n", type=int, default=8_000_000)
     ap.add_argument("--iters", type=int, default=10)
-    ap.add_argument("--variants", default="hash_learned_tuned,hash_tuned,hash_learned,hash_plain,sort_tuned,sort")
+    ap.add_argument("--variants", default="hash_learned_tuned,hash_tuned",
+                    help="comma-separated; the first is the value.  Others: hash_learned, hash_plain, sort, "
+                         "sort_tuned, *_graph (captured loop), *_ooc_sync/async")
     ap.add_argument("--buffer-pow", type=int, default=24,
                     help="out-of-core variants: batches of fewer than 2^b edges (-b)")
     ap.add_argument("--batch-graphs", type=int, default=512,
@@ -348,7 +445,8 @@ def main():
     dom_avg_s = dom["total_ms"] / dom["launches"] * 1e-3
     dom_bytes = dom["alg_bytes"] / dom["launches"]
     achieved = dom_bytes / dom_avg_s / 1e9
-    traffic = measured_traffic(workload, head, dom["name"])
+    src_hash = kernel_source_hash()
+    traffic, traffic_stale = measured_traffic(workload, head, dom["name"], src_hash)
     vg = {k: round(m * v["iters"] / v["wall_s"] / 1e9, 3) for k, v in variants.items()}
     # out-of-core: the neighbour array crosses the host link once per iteration
     link = {k: round(4.0 * m * v["iters"] / v["wall_s"] / 1e9, 2) for k, v in variants.items() if "_ooc_" in k}
@@ -375,7 +473,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "alg_bytes_per_launch": dom_bytes,
-                     **({"traffic_source": "profiles/hbm_traffic.json"} if traffic is not None else {}),
+                     "traffic_source": "lookup: profiles/hbm_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                                       "entries keyed by workload, variant, kernel and kernel-source hash)",
+                     "kernel_source_hash": src_hash, "traffic_stale": traffic_stale,
                      "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "iteration_roofline": {"alg_bytes": b_iter, "achieved_GBps": round(b_iter / per_iter_s / 1e9, 1),
                                "frac": round(b_iter / per_iter_s / 1e9 / HBM_PEAK_GBS, 4)},
@@ -406,17 +506,7 @@ def main():
         out["tuned_streams"] = main_v["tuned_streams"]
 
     if not args.no_cpu:
-        import oracle
-        cores = min(16, os.cpu_count() or 1)
-        t = time.perf_counter()
-        final, per, counts = oracle.lp_run(off, nbr, niter=args.iters, nthreads=cores, keep_iters=False)
-        cpu_s = time.perf_counter() - t
-        k = len(counts)
-        out["cpu_baseline"] = {"value": round(m * k / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": cores,
-                               "kind": "port",
-                               "sample": f"the full workload: {gname}, {k} iterations, "
-                                         f"{cpu_s:.2f} s wall on {cores} threads"}
-        out["labels_bit_exact"] = {name: bool(np.array_equal(v["labels"], final)) for name, v in variants.items()}
+        out.update(cpu_legs(off, nbr, gname, args.iters, variants))
     print(json.dumps(out), flush=True)
     ctx.close()
 

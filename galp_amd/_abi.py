@@ -120,6 +120,8 @@ def lib():
                                         ctypes.POINTER(ctypes.POINTER(HostCsr)),
                                         ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))]),
         "galp_host_csr_from_arrays": (ctypes.c_int, [i64, i64, vp, vp, ctypes.POINTER(ctypes.POINTER(HostCsr))]),
+        "galp_cpu_lp_run": (ctypes.c_int, [ctypes.POINTER(HostCsr), i32, vp, ctypes.POINTER(dbl),
+                                           ctypes.POINTER(i32)]),
         "galp_host_csr_free": (None, [ctypes.POINTER(HostCsr)]),
         "galp_host_free": (None, [vp]),
         "galp_host_last_error": (ctypes.c_char_p, []),
@@ -217,6 +219,19 @@ def save_bin(path, off, nbr):
     c = HostCsr(len(off) - 1, len(nbr), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                 nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
     _check_host(lib().galp_save_bin(os.fsencode(path), ctypes.byref(c)))
+
+
+def cpu_lp_run(off, nbr, niter=10):
+    """The serial CPU path (galp_cpu_lp_run): returns (labels, iterations, seconds)."""
+    off = np.ascontiguousarray(off, np.int64)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    c = HostCsr(len(off) - 1, len(nbr), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    out = np.empty(len(off) - 1, np.int32)
+    secs, iters = ctypes.c_double(), ctypes.c_int32()
+    _check_host(lib().galp_cpu_lp_run(ctypes.byref(c), int(niter), out.ctypes.data, ctypes.byref(secs),
+                                      ctypes.byref(iters)))
+    return out, iters.value, secs.value
 
 
 def accuracy(truth_file, labels):

@@ -8,9 +8,10 @@
 //   stdout "%d: %f (s), updated %d" per iteration   lfht.cuh:95
 //   stderr "<proc>\t<file>\t<niter>\t<b>\t<f1>\t<f2>\t<nmi>\t<fm>\t<ari>"  main.cu:240-242
 // Exit status 1 on any failure (the stop signal of experiments/common.sh:47).
-// Policies that need more than one GPU or the CPU (3, 9, 10, -g > 1) and the
-// serial CPU path (method*5+policy outside 0..10) are not part of this
-// one-GPU build: they print a message and exit 1.
+// Policies that need more than one GPU or the CPU (3, 8, 9, 10, -g > 1) are
+// not part of this one-GPU build: they print a message and exit 1.  A key
+// method*5+policy outside 0..10 runs the serial CPU path on the host
+// (galp_cpu_lp_run; main.cu:200-206), without opening the GPU.
 
 #include <getopt.h>
 #include <libgen.h>
@@ -50,6 +51,34 @@ static void usage(const char *prog) {
     std::exit(1);
 }
 
+// Accuracy (optional), the --labels file and the stderr TSV record of
+// main.cu:221-242.  Returns the exit status.
+static int finish(const std::string &proc_name, const char *graph_file, int niter, int buffer_pow, double f1,
+                  double f2, const char *truth_file, const std::vector<int32_t> &labels, const char *labels_file) {
+    double nmi = 0.0, fm = 0.0, ari = 0.0;
+    if (truth_file) {
+        if (galp_accuracy(truth_file, labels.data(), (int64_t)labels.size(), &nmi, &fm, &ari) != GALP_OK) {
+            std::printf("Cannot read the ground-truth file %s\n", truth_file);
+            return 1;
+        }
+    }
+    if (labels_file) {
+        FILE *f = std::fopen(labels_file, "w");
+        if (!f) {
+            std::printf("Cannot write %s\n", labels_file);
+            return 1;
+        }
+        for (int32_t l : labels) std::fprintf(f, "%d\n", l);
+        std::fclose(f);
+    }
+    std::string fname(graph_file);
+    std::vector<char> buf(fname.begin(), fname.end());
+    buf.push_back('\0');
+    std::fprintf(stderr, "%s\t%s\t%d\t%d\t%f\t%f\t%f\t%f\t%f\n", proc_name.c_str(), basename(buf.data()), niter,
+                 buffer_pow, f1, f2, nmi, fm, ari);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     int buffer_pow = 24, niter = 10, ngpus = 1, lfht_policy = 1;
     int dispatch = GALP_DISPATCH_PLAIN, reorder = 1;
@@ -83,21 +112,18 @@ int main(int argc, char **argv) {
     const char *graph_file = args[2];
 
     const int key = method * 5 + policy;
-    if (key < 0 || key > 10) {
-        std::printf("The serial CPU path (method*5+policy outside 0..10) is not part of the MI355X build.\n");
-        return 1;
-    }
+    const bool serial = key < 0 || key > 10;
     if (key == 3 || key == 8) {
         std::printf("Policy 3 (CPU-GPU hybrid) is not part of the one-GPU MI355X build.\n");
         return 1;
     }
-    if (key == 9 || key == 10 || ngpus != 1) {
+    if (key == 9 || key == 10 || (ngpus != 1 && !serial)) {
         std::printf("Multi-GPU runs are not part of the one-GPU MI355X build (-g must be 1).\n");
         return 1;
     }
 
     galp_ctx *ctx = nullptr;
-    if (galp_open(0, &ctx) != GALP_OK) {
+    if (!serial && galp_open(0, &ctx) != GALP_OK) {
         std::printf("No usable GPU (galp_open failed).\n");
         return 1;
     }
@@ -110,6 +136,18 @@ int main(int argc, char **argv) {
     std::printf("Data is ready on the host (%lld, %lld)\n", (long long)g->n, (long long)g->m);
 
     std::string proc_name;
+    if (serial) {   // main.cu:200-206
+        std::printf("----------serial\t ----------\n");
+        std::fflush(stdout);
+        std::vector<int32_t> labels((size_t)g->n);
+        double secs = 0.0;
+        if (galp_cpu_lp_run(g, niter, labels.data(), &secs, nullptr) != GALP_OK) {
+            std::printf("galp: %s\n", galp_host_last_error());
+            return 1;
+        }
+        return finish("serial\t ", graph_file, niter, 0, secs, secs, rest == 4 ? args[3] : nullptr, labels,
+                      labels_file);
+    }
     switch (key) {
     case 0: buffer_pow = 0; proc_name = "dpp\tincore"; break;
     case 1: proc_name = "dpp\tsync"; break;
@@ -178,29 +216,9 @@ int main(int argc, char **argv) {
         return 1;
     }
 
-    double nmi = 0.0, fm = 0.0, ari = 0.0;
-    if (rest == 4) {
-        if (galp_accuracy(args[3], labels.data(), (int64_t)labels.size(), &nmi, &fm, &ari) != GALP_OK) {
-            std::printf("Cannot read the ground-truth file %s\n", args[3]);
-            return 1;
-        }
-    }
-    if (labels_file) {
-        FILE *f = std::fopen(labels_file, "w");
-        if (!f) {
-            std::printf("Cannot write %s\n", labels_file);
-            return 1;
-        }
-        for (int32_t l : labels) std::fprintf(f, "%d\n", l);
-        std::fclose(f);
-    }
-
-    std::string fname(graph_file);
-    std::vector<char> buf(fname.begin(), fname.end());
-    buf.push_back('\0');
-    std::fprintf(stderr, "%s\t%s\t%d\t%d\t%f\t%f\t%f\t%f\t%f\n", proc_name.c_str(), basename(buf.data()), niter,
-                 buffer_pow, res.f1_s, res.f2_s, nmi, fm, ari);
-
+    const int frc = finish(proc_name, graph_file, niter, buffer_pow, res.f1_s, res.f2_s, rest == 4 ? args[3] : nullptr,
+                           labels, labels_file);
+    if (frc) return frc;
     galp_graph_free(dg);
     galp_host_csr_free(g);
     galp_close(ctx);

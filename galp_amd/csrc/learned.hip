@@ -94,8 +94,11 @@ __device__ __forceinline__ int64_t wave_append(int64_t *ctr, bool want) {
     return want ? (int64_t)base + __popcll(ball & ((1ull << lane_id()) - 1ull)) : -1;
 }
 
-__global__ void k_reset_ctrs(int64_t *ctr) {
-    if (threadIdx.x < kNumCtrs) ctr[threadIdx.x] = 0;
+// Zeroes ctr[0, count); count <= blockDim.x.  Every caller passes the
+// number of words its buffer holds.
+static_assert(kNumCtrs <= 64, "k_reset_ctrs runs one 64-thread block");
+__global__ void k_reset_ctrs(int64_t *ctr, int count) {
+    if ((int)threadIdx.x < count) ctr[threadIdx.x] = 0;
 }
 
 // Clears the dispatch's lists and counters.  first == 0 (out-of-core batches
@@ -787,7 +790,7 @@ int learn_setup(galp_graph *g) {
     k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_sel, g->nhubs, 1);
     k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_hd, g->nhubs, 0);
     k_fill_i32<<<1, 256, 0, g->stream>>>((int32_t *)L->d_hub_distinct, g->nhubs, 0);
-    k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_log_n);
+    k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_log_n, 1);   // d_log_n holds one word
     if (big_n && g->ooc) {
         GALP_LTRY(ooc_features(g));   // the rows stream through the batch buffers
     } else if (big_n) {
@@ -828,7 +831,7 @@ int learn_reset_iteration(galp_graph *g) {
         k_fill_i32<<<gd, 256, 0, g->stream>>>(L->d_dist[0], g->n, -1);
         k_fill_i32<<<1, 256, 0, g->stream>>>(L->d_hub_sel, g->nhubs, 1);
         k_fill_i32<<<grid_cap(g, (L->nmt + 255) / 256, 8), 256, 0, g->stream>>>(L->d_mt_cand, L->nmt, 0);
-        k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_plist_n);
+        k_reset_ctrs<<<1, 64, 0, g->stream>>>(L->d_plist_n, kNumCtrs);
     }
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "reset kernels failed");
 }

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "device_common.hpp"
@@ -138,16 +139,138 @@ __global__ void k_mlp_reduce(const float *__restrict__ pgrad, const float *__res
     }
 }
 
-// Adam (Kingma & Ba 2015) on every parameter.
-__global__ void k_mlp_adam(float *__restrict__ p, const float *__restrict__ grad, float *__restrict__ m,
-                           float *__restrict__ v, int P, float lr, float b1, float b2, float eps, float c1, float c2) {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < P; q += gridDim.x * blockDim.x) {
-        const float g = grad[q];
-        const float mq = b1 * m[q] + (1.f - b1) * g;
-        const float vq = b2 * v[q] + (1.f - b2) * g * g;
-        m[q] = mq;
-        v[q] = vq;
-        p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+// A whole training run in one launch of one workgroup: `epochs` passes over
+// the (already shuffled) samples in minibatches of B, each minibatch's
+// gradient of the mean loss, then one Adam step (Kingma & Ba 2015).  The
+// batch is walked in chunks of kTrainChunk samples: one thread per sample
+// runs the forward and backward pass and stages its activations and deltas
+// in LDS, then every thread adds the chunk's share to the gradient of the
+// parameters it owns (kept in registers across the chunks, summed in sample
+// order, so a run is deterministic).  The owners then apply Adam to their
+// parameters, with first and second moments held in registers for the
+// whole run, and publish them to LDS for the next minibatch.  One launch per
+// run replaces three per minibatch (gradient, block reduction, Adam).
+constexpr int kTrainThreads = 512;
+constexpr int kTrainChunk = 256;
+template <int H>
+__global__ void __launch_bounds__(kTrainThreads)
+k_mlp_train(float *__restrict__ params, const float *__restrict__ x, const float *__restrict__ y, int64_t n,
+            int epochs, int B, float lr, float b1, float b2, float eps) {
+    constexpr int P = mlp_num_params(H);
+    constexpr int S = kTrainChunk;
+    constexpr int Q = (P + kTrainThreads - 1) / kTrainThreads;   // parameters per thread
+    __shared__ float sp[P];
+    __shared__ float sx[S][4];
+    __shared__ float sh1[S][H + 1];   // +1: spread LDS banks
+    __shared__ float sh2[S][H + 1];
+    __shared__ float sd1[S][H + 1];
+    __shared__ float sd2[S][H + 1];
+    __shared__ float sg[S];
+    const int t = threadIdx.x;
+    float pm[Q], pv[Q], pw[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        const int q = t + r * kTrainThreads;
+        pm[r] = 0.f;
+        pv[r] = 0.f;
+        pw[r] = q < P ? params[q] : 0.f;
+        if (q < P) sp[q] = pw[r];
+    }
+    __syncthreads();
+    const float *W1 = sp, *bb1 = sp + 4 * H, *W2 = bb1 + H, *bb2 = W2 + H * H, *W3 = bb2 + H, *bb3 = W3 + H;
+    const float scale = 1.0f / (float)B;
+    float pow1 = 1.f, pow2 = 1.f;   // b1^step, b2^step
+    for (int e = 0; e < epochs; ++e) {
+        for (int64_t off = 0; off + B <= n; off += B) {
+            float acc[Q];
+#pragma unroll
+            for (int r = 0; r < Q; ++r) acc[r] = 0.f;
+            for (int c0 = 0; c0 < B; c0 += S) {
+                const int cs = min(S, B - c0);
+                if (t < cs) {   // forward and backward of sample off + c0 + t
+                    const int64_t s = off + c0 + t;
+                    float xs[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) xs[k] = x[4 * s + k];
+                    float h1[H], z2[H];
+#pragma unroll
+                    for (int j = 0; j < H; ++j) {
+                        const float a = bb1[j] + W1[4 * j] * xs[0] + W1[4 * j + 1] * xs[1] + W1[4 * j + 2] * xs[2] +
+                                        W1[4 * j + 3] * xs[3];
+                        h1[j] = a > 0.f ? a : 0.f;
+                    }
+                    float out = bb3[0];
+#pragma unroll 2
+                    for (int j = 0; j < H; ++j) {
+                        float a = bb2[j];
+#pragma unroll
+                        for (int k = 0; k < H; ++k) a += W2[H * j + k] * h1[k];
+                        z2[j] = a;
+                        out += W3[j] * (a > 0.f ? a : 0.f);
+                    }
+                    const float g = (out - y[s]) * scale;
+                    sg[t] = g;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) sx[t][k] = xs[k];
+                    for (int j = 0; j < H; ++j) {
+                        sh1[t][j] = h1[j];
+                        sh2[t][j] = z2[j] > 0.f ? z2[j] : 0.f;
+                        sd2[t][j] = z2[j] > 0.f ? g * W3[j] : 0.f;
+                    }
+#pragma unroll 2
+                    for (int k = 0; k < H; ++k) {
+                        float a = 0.f;
+#pragma unroll 8
+                        for (int j = 0; j < H; ++j) a += W2[H * j + k] * sd2[t][j];
+                        sd1[t][k] = h1[k] > 0.f ? a : 0.f;
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    const int q = t + r * kTrainThreads;
+                    float a = acc[r];
+                    if (q < 4 * H) {
+                        const int j = q / 4, k = q % 4;
+                        _Pragma("unroll 4") for (int i = 0; i < cs; ++i) a += sd1[i][j] * sx[i][k];
+                    } else if (q < 5 * H) {
+                        const int j = q - 4 * H;
+                        _Pragma("unroll 4") for (int i = 0; i < cs; ++i) a += sd1[i][j];
+                    } else if (q < 5 * H + H * H) {
+                        const int j = (q - 5 * H) / H, k = (q - 5 * H) % H;
+                        _Pragma("unroll 4") for (int i = 0; i < cs; ++i) a += sd2[i][j] * sh1[i][k];
+                    } else if (q < 6 * H + H * H) {
+                        const int j = q - 5 * H - H * H;
+                        _Pragma("unroll 4") for (int i = 0; i < cs; ++i) a += sd2[i][j];
+                    } else if (q < 7 * H + H * H) {
+                        const int k = q - 6 * H - H * H;
+                        _Pragma("unroll 4") for (int i = 0; i < cs; ++i) a += sg[i] * sh2[i][k];
+                    } else if (q < P) {
+                        _Pragma("unroll 4") for (int i = 0; i < cs; ++i) a += sg[i];
+                    }
+                    acc[r] = a;
+                }
+                __syncthreads();   // the chunk's LDS rows are rewritten next
+            }
+            pow1 *= b1;
+            pow2 *= b2;
+            const float c1 = 1.f - pow1, c2 = 1.f - pow2;
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                const int q = t + r * kTrainThreads;
+                const float g = acc[r];
+                pm[r] = b1 * pm[r] + (1.f - b1) * g;
+                pv[r] = b2 * pv[r] + (1.f - b2) * g * g;
+                pw[r] -= lr * (pm[r] / c1) / (sqrtf(pv[r] / c2) + eps);
+                if (q < P) sp[q] = pw[r];
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        const int q = t + r * kTrainThreads;
+        if (q < P) params[q] = pw[r];
     }
 }
 
@@ -169,6 +292,18 @@ int mlp_predict_dev(hipStream_t st, int H, const float *dparams, const float *dx
     case 8: k_mlp_predict<8><<<grid, 256, 0, st>>>(dparams, dx, n, dy); break;
     case 16: k_mlp_predict<16><<<grid, 256, 0, st>>>(dparams, dx, n, dy); break;
     case 32: k_mlp_predict<32><<<grid, 256, 0, st>>>(dparams, dx, n, dy); break;
+    default: return GALP_EINVAL;
+    }
+    return hipGetLastError() == hipSuccess ? GALP_OK : GALP_EHIP;
+}
+
+static int mlp_train_dev(hipStream_t st, int H, float *dp, const float *dx, const float *dy, int64_t n, int epochs,
+                         int B, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    switch (H) {
+    case 8: k_mlp_train<8><<<1, kTrainThreads, 0, st>>>(dp, dx, dy, n, epochs, B, lr, b1, b2, eps); break;
+    case 16: k_mlp_train<16><<<1, kTrainThreads, 0, st>>>(dp, dx, dy, n, epochs, B, lr, b1, b2, eps); break;
+    case 32: k_mlp_train<32><<<1, kTrainThreads, 0, st>>>(dp, dx, dy, n, epochs, B, lr, b1, b2, eps); break;
     default: return GALP_EINVAL;
     }
     return hipGetLastError() == hipSuccess ? GALP_OK : GALP_EHIP;
@@ -205,7 +340,10 @@ using namespace galp;
     do {                                                                                        \
         hipError_t _e = (call);                                                                 \
         if (_e != hipSuccess) {                                                                 \
-            if (st) (void)hipStreamDestroy(st);                                                 \
+            if (st) {   /* drain queued work before the buffers are freed */                       \
+                (void)hipStreamSynchronize(st);                                                 \
+                (void)hipStreamDestroy(st);                                                     \
+            }                                                                                   \
             return fail(ctx, GALP_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));     \
         }                                                                                       \
     } while (0)
@@ -314,36 +452,18 @@ GALP_API int galp_mlp_train(galp_ctx *ctx, int32_t hidden, float *params, const 
         ys[(size_t)i] = y[order[(size_t)i]];
     }
     const int64_t B = std::min<int64_t>(batch, n);
-    const int64_t nb = (B + 63) / 64;
-    DevBuf dp, dx, dy, dpg, dpl, dg, dl, dm, dv;
+    DevBuf dp, dx, dy, dg, dl;
     GALP_CK(dp.alloc(4 * P));
-    GALP_CK(dm.alloc(4 * P));
-    GALP_CK(dv.alloc(4 * P));
     GALP_CK(dx.alloc(16 * (size_t)n));
     GALP_CK(dy.alloc(4 * (size_t)n));
-    GALP_CK(dpg.alloc(4 * (size_t)(nb * P)));
-    GALP_CK(dpl.alloc(4 * (size_t)nb));
     GALP_CK(dg.alloc(4 * P));
     GALP_CK(dl.alloc(4));
     GALP_CK(hipMemcpyAsync(dp.p, params, 4 * P, hipMemcpyHostToDevice, st));
-    GALP_CK(hipMemsetAsync(dm.p, 0, 4 * P, st));
-    GALP_CK(hipMemsetAsync(dv.p, 0, 4 * P, st));
     GALP_CK(hipMemcpyAsync(dx.p, xs.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
     GALP_CK(hipMemcpyAsync(dy.p, ys.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
-    int64_t step = 0;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int e = 0; e < epochs; ++e) {
-        for (int64_t off = 0; off + B <= n; off += B) {
-            ++step;
-            if (mlp_grad_dev(st, hidden, dp.as<float>(), dx.as<float>() + 4 * off, dy.as<float>() + off, B,
-                             dpg.as<float>(), dpl.as<float>(), dg.as<float>(), dl.as<float>()) != GALP_OK)
-                GALP_CK(hipGetLastError());
-            const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
-            k_mlp_adam<<<(int)((P + 255) / 256), 256, 0, st>>>(dp.as<float>(), dg.as<float>(), dm.as<float>(),
-                                                               dv.as<float>(), (int)P, lr, b1, b2, eps, c1, c2);
-            GALP_CK(hipGetLastError());
-        }
-    }
+    if (epochs > 0 && mlp_train_dev(st, hidden, dp.as<float>(), dx.as<float>(), dy.as<float>(), n, epochs, (int)B,
+                                    lr) != GALP_OK)
+        GALP_CK(hipGetLastError());
     // loss of the final parameters over the whole set
     {
         const int64_t nbf = (n + 63) / 64;

@@ -116,6 +116,19 @@ GALP_API int galp_accuracy(const char *truth_file, const int32_t *labels,
                            int64_t n, double *nmi, double *fmeasure,
                            double *ari);
 
+/* The serial CPU path: labels[i] = i, then up to niter synchronous
+ * iterations with the n*1e-5 stop rule, on one host thread; the final labels
+ * go to labels_out[n], the wall time to *seconds and the iterations run to
+ * *iterations (both may be NULL).  Replaces LabelPropagator::run
+ * (label_propagation/include/label_propagator.h:37-82), which main.cu:200-206
+ * selects for method*5+policy outside 0..10.  Same update and tie rule as the
+ * GPU methods (galp.h), so the labels equal theirs; the reference's serial
+ * path instead updates in place in a shuffled order and flips a coin on ties
+ * (label_propagator.h:50,61-62). */
+GALP_API int galp_cpu_lp_run(const galp_host_csr *g, int32_t niter,
+                             int32_t *labels_out, double *seconds,
+                             int32_t *iterations);
+
 GALP_API void galp_host_csr_free(galp_host_csr *g);
 GALP_API void galp_host_free(void *p);
 

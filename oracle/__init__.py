@@ -16,7 +16,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_BASE_PATH = os.path.join(_HERE, "_build", "libbaseline.so")
 _lib = None
+_base = None
 
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -57,6 +59,56 @@ def lib():
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+def baseline_lib():
+    """lp_baseline.cpp: the std::map-structured CPU baseline."""
+    global _base
+    if _base is None:
+        if not os.path.exists(_BASE_PATH):
+            build()
+        L = ctypes.CDLL(_BASE_PATH)
+        L.baseline_map_iterate.restype = ctypes.c_int64
+        L.baseline_map_iterate.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, _i32p,
+                                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
+        L.baseline_max_threads.restype = ctypes.c_int
+        L.baseline_max_threads.argtypes = []
+        _base = L
+    return _base
+
+
+def map_iterate(off, nbr, prev, vlist=None, nthreads=0, out=None):
+    """One synchronous iteration of the std::map-structured baseline over the
+    vertices in vlist (int64; all when None).  Returns (next, changed); next
+    holds prev's value at vertices outside vlist."""
+    off, nbr = _csr(off, nbr)
+    prev = np.ascontiguousarray(prev, dtype=np.int32)
+    n = len(off) - 1
+    nxt = prev.copy() if out is None else out
+    if vlist is not None:
+        vlist = np.ascontiguousarray(vlist, dtype=np.int64)
+    nv = n if vlist is None else len(vlist)
+    c = baseline_lib().baseline_map_iterate(n, off, nbr, prev, nxt,
+                                            vlist.ctypes.data if vlist is not None else None,
+                                            nv, int(nthreads))
+    return nxt, int(c)
+
+
+def baseline_threads():
+    return int(baseline_lib().baseline_max_threads())
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
 
 
 def _csr(off, nbr):

@@ -1,5 +1,6 @@
 """bench.py's contract pieces that need no GPU: the refusal of more than one
-GPU and the lookup of measured HBM traffic for the roofline line."""
+GPU and the lookup of measured HBM traffic for the roofline line, which only
+returns a figure measured on the current kernel sources."""
 import json
 import os
 import subprocess
@@ -21,14 +22,27 @@ def test_more_than_one_gpu_is_refused():
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
 
 
-def test_measured_traffic_lookup():
+def test_measured_traffic_lookup(tmp_path, monkeypatch):
+    f = tmp_path / "traffic.json"
+    entry = {"workload": RMAT22, "variant": "hash_learned_tuned", "kernel": "cand", "bytes_per_launch": 900,
+             "algorithmic_bytes_per_launch": 600, "source_hash": "abc"}
+    f.write_text(json.dumps({"entries": [entry]}))
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(f))
+    assert bench.measured_traffic(RMAT22, "hash_learned_tuned", "cand", "abc") == (900.0, False)
+    # measured on other kernel sources: withheld and flagged
+    assert bench.measured_traffic(RMAT22, "hash_learned_tuned", "cand", "def") == (None, True)
+    assert bench.measured_traffic(RMAT22, "hash_plain", "no-such-kernel", "abc") == (None, False)
+    assert bench.measured_traffic("another workload", "hash_learned_tuned", "cand", "abc") == (None, False)
+
+
+def test_committed_traffic_entries_carry_a_source_hash():
     with open(bench.TRAFFIC_FILE) as f:
         entries = json.load(f)["entries"]
-    cand = [e for e in entries if e["workload"] == RMAT22 and e["variant"] == "hash_learned_tuned"
-            and e["kernel"] == "cand"]
-    assert len(cand) == 1
-    got = bench.measured_traffic(RMAT22, "hash_learned_tuned", "cand")
-    assert got == float(cand[0]["bytes_per_launch"])
-    assert got > cand[0]["algorithmic_bytes_per_launch"] > 0   # traffic is at least the algorithmic bytes
-    assert bench.measured_traffic(RMAT22, "hash_plain", "no-such-kernel") is None
-    assert bench.measured_traffic("another workload", "hash_learned_tuned", "cand") is None
+    for e in entries:
+        assert e["bytes_per_launch"] >= e["algorithmic_bytes_per_launch"] > 0
+        assert len(e.get("source_hash", "")) == 16, e
+
+
+def test_kernel_source_hash_is_stable():
+    assert bench.kernel_source_hash() == bench.kernel_source_hash()
+    assert len(bench.kernel_source_hash()) == 16
