@@ -349,10 +349,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
                                            2 * esz * (n + 1)
                                      : 4 * m;
         int64_t need = resident + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
-        if (g->opts.method == GALP_METHOD_HASH)
-            need += 8 * g->nhub_items + (8 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
-        else   // compact hub rows: keys and positions, twice; histograms and their scan
-            need += 16 * g->bucket[B_HUB].edges + 12 * 256 * g->nhub_tiles;
+        // the hub plan of both methods: merge items, tile outputs and partition offsets
+        need += 8 * g->nhub_items + (8 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
         if (g->reorder) need += 8 * n + (ooc ? 0 : 4 * m + 8 * (n + 1));  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;
@@ -434,7 +432,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_HIP(ctx, up(g->d_hub_tile0, hub_tile0.data(), 4 * hub_tile0.size()));
         GALP_HIP(ctx, up(g->d_hub_ntiles, hub_ntiles.data(), 4 * hub_ntiles.size()));
         GALP_HIP(ctx, up(g->d_tile_hub, tile_hub.data(), 4 * tile_hub.size()));
-        if (g->opts.method == GALP_METHOD_HASH) {   // tile outputs and merge items of the hash hub path
+        {   // tile outputs and merge items of both methods' hub paths
             GALP_TRY(dev_alloc(g, &g->d_item_hub, g->nhub_items));
             GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
             GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
@@ -453,11 +451,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         for (size_t h = 0; h < hub_nparts.size(); ++h) hub_item0[h + 1] = hub_item0[h] + hub_nparts[h];
         GALP_TRY(ooc_setup(g, c, offi, perm, iperm, vlist, hub_vertices, hub_tile0, hub_item0));
     }
-    if (g->opts.method == GALP_METHOD_SORT) {
-        std::vector<int64_t> hub_deg;
-        for (int32_t x : hub_vertices) hub_deg.push_back(offi[(size_t)x + 1] - offi[(size_t)x]);
-        GALP_TRY(sort_setup(g, hub_deg));
-    }
+    if (g->opts.method == GALP_METHOD_SORT) GALP_TRY(sort_setup(g));
     GALP_HIP(ctx, hipStreamSynchronize(g->stream));
     if (g->opts.dispatch == GALP_DISPATCH_LEARNED) GALP_TRY(learn_setup(g));
     g->preprocess_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

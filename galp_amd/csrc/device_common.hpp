@@ -20,6 +20,13 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t key, int bits) {
     return (key * 0x9E3779B1u) >> (32 - bits);
 }
 
+// Label partition of a hub's label (P partitions): a multiplicative hash
+// scaled to [0, P), so a hub's distinct labels spread evenly over its
+// partitions whatever their values.  Both methods' hub paths use it.
+__device__ __forceinline__ uint32_t hub_partition(uint32_t L, uint32_t P) {
+    return (uint32_t)(((unsigned long long)(L * 0x85EBCA77u) * P) >> 32);
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
 // ---------------------------------------------------------------------------

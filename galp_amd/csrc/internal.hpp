@@ -160,18 +160,10 @@ struct LearnState {
     int64_t *d_log_n = nullptr;
     int32_t *distinct_out(const galp_graph *) const { return d_dist[dcur ^ 1]; }
 };
-// State of the sort method's hub path: a compact buffer of the hub rows
-// (hub h at [hoff[h], hoff[h+1])), ping-pong keys and positions, and the
-// per-(hub, digit, tile) histograms and their scan.
+// State of the sort method: the 8-bit radix passes its LDS sorts make for
+// labels < n.  Its hub path uses the hub plan both methods share.
 struct SortState {
-    int passes = 0;                    // 8-bit LSD passes for labels < n
-    int64_t eh = 0;                    // hub edges
-    int64_t *d_hoff = nullptr;
-    uint32_t *d_kA = nullptr, *d_kB = nullptr, *d_pA = nullptr, *d_pB = nullptr;
-    uint32_t *d_hist = nullptr;
-    int64_t *d_scan = nullptr, *d_bsum = nullptr;
-    int64_t nscan = 0, nbsum = 0;
-    std::vector<int64_t> hoff_host;
+    int passes = 0;
 };
 
 
@@ -321,7 +313,7 @@ int side_join(galp_graph *g, int k);
 int launch_hash_iteration_incore(galp_graph *g);
 
 // Sort method (lp_sort.hip).
-int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg);
+int sort_setup(galp_graph *g);
 void sort_free(galp_graph *g);
 int launch_sort_iteration(galp_graph *g, const IterView &w);
 int launch_sort_class(galp_graph *g, int b);   // one class of the whole graph (the autotuner's unit)

@@ -331,9 +331,6 @@ k_hash_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr,
 //     table and no atomics on it -- and applies the tie rule within the
 //     partition; one 64-bit atomicMax of (count << 32 | last) per block
 //     picks the hub's winner, which is the label at position `last`.
-__device__ __forceinline__ uint32_t hub_partition(uint32_t L, uint32_t P) {
-    return (uint32_t)(((unsigned long long)(L * 0x85EBCA77u) * P) >> 32);
-}
 
 // HD: the launch for high-distinct hubs (lane-by-lane inserts, no peel
 // rounds); with a hub_hd list, each launch takes the hubs of its kind.

@@ -20,10 +20,10 @@
 //     alternative; 1025..4096: an LDS LSD radix sort of the labels (8-bit digits, stable
 //     multisplit by ballots, positions as payload); run lengths by binary
 //     search for each run's first element;
-//   - hubs (> 4096): a segmented LSD radix sort (8-bit digits, as many passes
-//     as the label width needs) of the hub edges in a compact buffer, then a
-//     per-tile run-length pass; tiles reduce (count, last) with one 64-bit
-//     atomicMax per tile per hub.
+//   - hubs (> 4096): each 2048-edge tile sorted in LDS and run-length encoded
+//     into (label, count, last) entries bucketed by label partition, then one
+//     block per (hub, partition) sorts that partition's entries of every tile
+//     and reduces each run; one 64-bit atomicMax of (count, last) per block.
 // The sort key is (label, position in the row), so every sort is stable in
 // the row order and the last element of a run holds the run's largest
 // position: the winner argmax (count, last) equals the hash method's.
@@ -424,12 +424,101 @@ __device__ __forceinline__ unsigned long long digit_peers8(uint32_t d, bool vali
     return peers;
 }
 
+// Stable LSD radix sort of skey[0, len) by `passes` 8-bit digits, with the
+// 16-bit payload spos[0, len) riding along, in LDS; each array has two
+// halves of N entries (ping-pong).  All T threads of the block call it;
+// every wave owns the contiguous elements [w*M, (w+1)*M) (M = N / waves),
+// 8 ballots give each lane the lanes of its 64-element chunk with the same
+// digit, per-wave digit histograms (whist: waves x 256) and one scan over
+// (digit, wave) make each pass stable across the block.  Returns the half
+// (0 or 1) that holds the sorted keys; ends with a barrier.
+template <int T, int N>
+__device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, int len, int passes) {
+    constexpr int NW = T / kWave;
+    constexpr int M = N / NW;        // elements per wave
+    constexpr int CH = M / kWave;    // 64-element chunks per wave
+    static_assert(M % kWave == 0, "whole chunks per wave");
+    const int tid = (int)threadIdx.x;
+    const int lane = lane_id();
+    const int w = tid / kWave;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int src = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        const uint32_t *ks = skey + src * N;
+        const uint16_t *ps = spos + src * N;
+        uint32_t *kd = skey + (src ^ 1) * N;
+        uint16_t *pd = spos + (src ^ 1) * N;
+        for (int i = lane; i < 256; i += kWave) whist[w * 256 + i] = 0;
+        __syncthreads();   // keys written; histogram rows clear
+        // A: per-wave digit counts (the lowest lane of each peer group adds)
+        for (int c = 0; c < CH; ++c) {
+            const int j = w * M + c * kWave + lane;
+            const bool valid = j < len;
+            const uint32_t d = valid ? (ks[j] >> shift) & 0xffu : 0u;
+            const unsigned long long peers = digit_peers8(d, valid);
+            if (valid && (peers & lt) == 0) whist[w * 256 + d] += (uint32_t)__popcll(peers);
+            if (w * M + c * kWave >= len) break;   // uniform per wave
+        }
+        __syncthreads();
+        // bases: digit-major exclusive scan over (digit, wave)
+        if (w == 0) {
+            uint32_t tot[4];
+            uint32_t run = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {   // lane owns digits 4*lane .. 4*lane+3
+                uint32_t t = 0;
+                for (int ww = 0; ww < NW; ++ww) t += whist[ww * 256 + 4 * lane + q];
+                tot[q] = t;
+                run += t;
+            }
+            uint32_t inc = run;
+#pragma unroll
+            for (int sft = 1; sft < kWave; sft <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
+                if (lane >= sft) inc += y;
+            }
+            uint32_t base = inc - run;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int d = 4 * lane + q;
+                uint32_t b2 = base;
+                for (int ww = 0; ww < NW; ++ww) {
+                    const uint32_t cnt = whist[ww * 256 + d];
+                    whist[ww * 256 + d] = b2;
+                    b2 += cnt;
+                }
+                base += tot[q];
+            }
+        }
+        __syncthreads();
+        // B: stable scatter, each wave in chunk order with its running bases
+        for (int c = 0; c < CH; ++c) {
+            if (w * M + c * kWave >= len) break;   // uniform per wave
+            const int j = w * M + c * kWave + lane;
+            const bool valid = j < len;
+            const uint32_t key = valid ? ks[j] : 0u;
+            const uint32_t d = (key >> shift) & 0xffu;
+            const unsigned long long peers = digit_peers8(d, valid);
+            const uint32_t base = valid ? whist[w * 256 + d] : 0u;
+            if (valid) {
+                const uint32_t dst = base + (uint32_t)__popcll(peers & lt);
+                kd[dst] = key;
+                pd[dst] = ps[j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (valid && (peers & lt) == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
+            __builtin_amdgcn_wave_barrier();
+        }
+        src ^= 1;
+        __syncthreads();
+    }
+    return src;
+}
+
 template <int T, int EPT, typename E>
 struct LdsSortPolicy {
     static constexpr int N = T * EPT;
-    static constexpr int NW = T / kWave;
-    static constexpr int M = N / NW;        // elements per wave
-    static constexpr int CH = M / kWave;    // 64-element chunks per wave
     struct Item {
         bool valid;
         int len;
@@ -471,9 +560,6 @@ struct LdsSortPolicy {
         if (it.len == 0) return;
         const int len = it.len;
         const int tid = (int)threadIdx.x;
-        const int lane = lane_id();
-        const int w = tid / kWave;
-        const unsigned long long lt = (1ull << lane) - 1ull;
         __syncthreads();   // the previous vertex's readers are done
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
@@ -483,77 +569,7 @@ struct LdsSortPolicy {
                 spos[pos] = (uint16_t)pos;
             }
         }
-        int src = 0;
-        for (int p = 0; p < passes; ++p) {
-            const int shift = 8 * p;
-            const uint32_t *ks = skey + src * N;
-            const uint16_t *ps = spos + src * N;
-            uint32_t *kd = skey + (src ^ 1) * N;
-            uint16_t *pd = spos + (src ^ 1) * N;
-            for (int i = lane; i < 256; i += kWave) whist[w * 256 + i] = 0;
-            __syncthreads();   // keys written; histogram rows clear
-            // A: per-wave digit counts (the lowest lane of each peer group adds)
-            for (int c = 0; c < CH; ++c) {
-                const int j = w * M + c * kWave + lane;
-                const bool valid = j < len;
-                const uint32_t d = valid ? (ks[j] >> shift) & 0xffu : 0u;
-                const unsigned long long peers = digit_peers8(d, valid);
-                if (valid && (peers & lt) == 0) whist[w * 256 + d] += (uint32_t)__popcll(peers);
-                if (w * M + c * kWave >= len) break;   // uniform per wave
-            }
-            __syncthreads();
-            // bases: digit-major exclusive scan over (digit, wave)
-            if (w == 0) {
-                uint32_t tot[4];
-                uint32_t run = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {   // lane owns digits 4*lane .. 4*lane+3
-                    uint32_t t = 0;
-                    for (int ww = 0; ww < NW; ++ww) t += whist[ww * 256 + 4 * lane + q];
-                    tot[q] = t;
-                    run += t;
-                }
-                uint32_t inc = run;
-#pragma unroll
-                for (int sft = 1; sft < kWave; sft <<= 1) {
-                    const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
-                    if (lane >= sft) inc += y;
-                }
-                uint32_t base = inc - run;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int d = 4 * lane + q;
-                    uint32_t b2 = base;
-                    for (int ww = 0; ww < NW; ++ww) {
-                        const uint32_t cnt = whist[ww * 256 + d];
-                        whist[ww * 256 + d] = b2;
-                        b2 += cnt;
-                    }
-                    base += tot[q];
-                }
-            }
-            __syncthreads();
-            // B: stable scatter, each wave in chunk order with its running bases
-            for (int c = 0; c < CH; ++c) {
-                if (w * M + c * kWave >= len) break;   // uniform per wave
-                const int j = w * M + c * kWave + lane;
-                const bool valid = j < len;
-                const uint32_t key = valid ? ks[j] : 0u;
-                const uint32_t d = (key >> shift) & 0xffu;
-                const unsigned long long peers = digit_peers8(d, valid);
-                const uint32_t base = valid ? whist[w * 256 + d] : 0u;
-                if (valid) {
-                    const uint32_t dst = base + (uint32_t)__popcll(peers & lt);
-                    kd[dst] = key;
-                    pd[dst] = ps[j];
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (valid && (peers & lt) == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
-                __builtin_amdgcn_wave_barrier();
-            }
-            src ^= 1;
-            __syncthreads();
-        }
+        const int src = lds_radix_sort<T, N>(skey, spos, whist, len, passes);
         const uint32_t *kf = skey + src * N;
         const uint16_t *pf = spos + src * N;
         unsigned long long best = 0;
@@ -604,364 +620,327 @@ k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 }
 
 // ---------------------------------------------------------------------------
-// Hubs.  The edges of hub h occupy [hoff[h], hoff[h+1]) of a compact key
-// buffer, in kHubTile-edge tiles (the hub path's tile list: tiles of a hub
-// are consecutive).  Each LSD pass p sorts by bits [8p, 8p+8) of the label:
-//   hist:    per tile, the 256-digit histogram, stored hub-major, then
-//            digit-major, then tile: idx = tile0[h]*256 + d*ntiles[h] + tl;
-//   scan:    one exclusive scan over that array gives, for every (hub,
-//            digit, tile), the tile's first output slot -- hub-major order
-//            keeps every hub inside its own range, so the sort is segmented;
-//   scatter: per tile, a stable multisplit: wave w owns elements
-//            [512w, 512w+512) in order; 8 ballots give each lane the lanes
-//            of its chunk with the same digit, whose popcount below it is
-//            its rank; per-wave running counters in LDS carry the ranks
-//            across chunks.
-// The payload is the element's position in its row.  Pass 0 gathers the
-// labels (prev[nbr[e]]) into the key buffer and takes the position from the
-// index.
-constexpr int kSortTileT = 256;
-constexpr int kSortEPT = kHubTile / kSortTileT;   // 8
-static_assert(kHubTile == 2048 && kSortEPT == 8, "hub tile layout");
+// Hubs (degree > 4096), in the hub plan both methods share (tiles of
+// kHubTile edges, P_h = ceil(deg / kHubPart) label partitions per hub, merge
+// items (hub, partition)); every hub edge crosses HBM as a neighbour id and a
+// gathered label once, and the tiles' run-length output (12 bytes per
+// distinct label of a tile) once more each way.
+//  1. tile (k_hs_tile): a block sorts one tile's (label, position) pairs by
+//     an LDS radix sort (stable, so equal labels keep row order), turns each
+//     run into (label, count, last position) and buckets the runs by label
+//     partition (counting sort in LDS), writing them with the P_h + 1
+//     partition offsets -- the same layout as the hash method's tiles;
+//  2. merge (k_hs_merge): a block per (hub, partition) stages that
+//     partition's entries of every tile in LDS, tiles in row order, sorts
+//     them by label (stable radix sort, entry index as payload) and reduces
+//     each run: count = the sum of its counts (a block prefix sum), last =
+//     the last entry's position (each tile adds at most one entry per label
+//     and tiles are in row order, so that is the largest).  More entries than
+//     the LDS holds are merged in rounds: each round's reduced runs stay at
+//     the front and the next entries are appended behind them.  One 64-bit
+//     atomicMax of (count << 32 | last) per block picks the hub's winner;
+//  3. final (k_hs_final): the winner is the label at position `last`.
+constexpr int kHsTileT = 512;
+constexpr int kHsTileEPT = kHubTile / kHsTileT;   // 4
+constexpr int kHsMergeT = 1024;
+constexpr int kHsMergeCap = 4096;                 // entries staged per round
+static_assert(kHubTile == 2048 && kHsTileEPT == 4, "hub tile layout");
 
-struct HubTile {
-    int h;
-    int64_t tl;      // tile index within its hub
-    int64_t c0;      // first compact slot of the tile
-    int len;         // elements in the tile
-    int64_t hbase;   // tile0[h] * 256
-    int64_t nth;     // tiles of the hub
-};
+template <int T, typename E>
+struct HubSortTilePolicy {
+    static constexpr int EPT = kHubTile / T;
+    struct Item {
+        bool valid;
+        int len;
+        int64_t e0;       // absolute index of the tile's first edge
+        uint32_t row0;    // position of that edge in its row
+        uint32_t P;       // partitions of the hub
+        int64_t idx;      // offset of the tile's partition offsets
+        int64_t t;        // tile id
+    };
+    const E *off;
+    const int32_t *tile_hub, *hub_tile0, *hub_vid, *hub_nparts;
+    const int64_t *tile_idx_off;
+    int64_t ntiles;
+    uint32_t *skey;       // 2 x kHubTile
+    uint16_t *spos;       // 2 x kHubTile
+    uint32_t *whist;      // waves x 256
+    uint32_t *hist;       // kHubMaxParts + 1
+    int *sscan;
+    uint32_t *out_key, *out_cnt, *out_last;
+    int32_t *tile_idx;
+    int passes;
 
-__device__ __forceinline__ HubTile hub_tile_at(int64_t t, const int32_t *tile_hub, const int32_t *hub_tile0,
-                                               const int32_t *hub_ntiles, const int64_t *hoff) {
-    HubTile x;
-    x.h = tile_hub[t];
-    const int64_t t0 = hub_tile0[x.h];
-    x.tl = t - t0;
-    x.nth = hub_ntiles[x.h];
-    x.hbase = t0 * 256;
-    const int64_t hb = hoff[x.h], he = hoff[x.h + 1];
-    x.c0 = hb + x.tl * kHubTile;
-    x.len = (int)min((int64_t)kHubTile, he - x.c0);
-    return x;
-}
-
-// Pass-0 histogram; also gathers the labels into the key buffer.
-template <typename E>
-__global__ void __launch_bounds__(kSortTileT)
-k_hs_gather_hist(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
-                 const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ tile_hub,
-                 const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
-                 const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles, uint32_t *__restrict__ keys,
-                 uint32_t *__restrict__ hist, const int32_t *__restrict__ stop) {
-    __shared__ uint32_t sh[256];
-    if (*stop) return;
-    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
-        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
-        const int64_t rb = (int64_t)off[hub_vid[x.h]] + x.tl * kHubTile;
-        sh[threadIdx.x] = 0;
-        __syncthreads();
-        int32_t u[kSortEPT];
-#pragma unroll
-        for (int k = 0; k < kSortEPT; ++k) {
-            const int j = k * kSortTileT + (int)threadIdx.x;
-            u[k] = __builtin_nontemporal_load(&nbr[rb + (j < x.len ? j : 0)]);
+    __device__ Item item(int64_t t) const {
+        Item it{false, 0, 0, 0, 1, 0, t};
+        if (t < ntiles) {
+            const int h = tile_hub[t];
+            const int64_t v = hub_vid[h];
+            const int64_t rb = (int64_t)off[v], re = (int64_t)off[v + 1];
+            it.row0 = (uint32_t)((t - (int64_t)hub_tile0[h]) * (int64_t)kHubTile);   // tiles of a hub are consecutive
+            it.e0 = rb + it.row0;
+            it.len = (int)min((int64_t)kHubTile, re - it.e0);
+            it.P = (uint32_t)hub_nparts[h];
+            it.idx = tile_idx_off[t];
+            it.valid = true;
         }
+        return it;
+    }
+    __device__ int64_t edge0(const Item &it) const { return it.e0; }
+
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
+        const int len = it.len;
+        const int tid = (int)threadIdx.x;
+        __syncthreads();   // the previous tile's readers are done
 #pragma unroll
-        for (int k = 0; k < kSortEPT; ++k) {
-            const int j = k * kSortTileT + (int)threadIdx.x;
-            const uint32_t L = (uint32_t)prev[u[k]];
-            if (j < x.len) {
-                keys[x.c0 + j] = L;
-                atomicAdd(&sh[L & 0xffu], 1u);
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * T + tid;
+            if (pos < len) {
+                skey[pos] = (uint32_t)lab[k];
+                spos[pos] = (uint16_t)pos;
             }
         }
-        __syncthreads();
-        hist[x.hbase + (int64_t)threadIdx.x * x.nth + x.tl] = sh[threadIdx.x];
-        __syncthreads();
-    }
-}
-
-// Histogram of digit `shift` for passes >= 1.
-__global__ void __launch_bounds__(kSortTileT)
-k_hs_hist(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles,
-          const uint32_t *__restrict__ keys, int shift, uint32_t *__restrict__ hist,
-          const int32_t *__restrict__ stop) {
-    __shared__ uint32_t sh[256];
-    if (*stop) return;
-    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
-        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
-        sh[threadIdx.x] = 0;
-        __syncthreads();
-        uint32_t kv[kSortEPT];
+        const int src = lds_radix_sort<T, kHubTile>(skey, spos, whist, len, passes);
+        const uint32_t *kf = skey + src * kHubTile;
+        const uint16_t *pf = spos + src * kHubTile;
+        // runs: the thread holding a run's end finds its start by binary search
+        uint32_t key[EPT], cnt[EPT], last[EPT], part[EPT];
+        bool run_end[EPT];
 #pragma unroll
-        for (int k = 0; k < kSortEPT; ++k) {
-            const int j = k * kSortTileT + (int)threadIdx.x;
-            kv[k] = keys[x.c0 + (j < x.len ? j : 0)];
-        }
-#pragma unroll
-        for (int k = 0; k < kSortEPT; ++k) {
-            const int j = k * kSortTileT + (int)threadIdx.x;
-            if (j < x.len) atomicAdd(&sh[(kv[k] >> shift) & 0xffu], 1u);
-        }
-        __syncthreads();
-        hist[x.hbase + (int64_t)threadIdx.x * x.nth + x.tl] = sh[threadIdx.x];
-        __syncthreads();
-    }
-}
-
-// Exclusive scan of n uint32 into int64, in three launches: per-chunk sums,
-// a one-block scan of the sums, per-chunk scans from their bases.
-constexpr int kScanT = 256;
-constexpr int kScanPer = 16;
-constexpr int kScanChunk = kScanT * kScanPer;   // 4096
-
-template <int NW>
-__device__ __forceinline__ int64_t block_exclusive_sum(int64_t x, int64_t *sw, int64_t *total) {
-    const int lane = lane_id();
-    const int w = (int)(threadIdx.x / kWave);
-    int64_t inc = x;
-#pragma unroll
-    for (int s = 1; s < kWave; s <<= 1) {
-        const int64_t y = __shfl_up(inc, s);
-        if (lane >= s) inc += y;
-    }
-    if (lane == kWave - 1) sw[w] = inc;
-    __syncthreads();
-    int64_t before = 0, all = 0;
-#pragma unroll
-    for (int i = 0; i < NW; ++i) {
-        if (i < w) before += sw[i];
-        all += sw[i];
-    }
-    __syncthreads();
-    *total = all;
-    return before + inc - x;
-}
-
-__global__ void __launch_bounds__(kScanT)
-k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, int64_t *__restrict__ bsum, const int32_t *__restrict__ stop) {
-    __shared__ unsigned long long sred64[kScanT / kWave];
-    if (*stop) return;
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
-    unsigned long long s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        const int64_t i = base + (int64_t)k * kScanT + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane_id() == 0) sred64[threadIdx.x / kWave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < kScanT / kWave; ++i) t += sred64[i];
-        bsum[blockIdx.x] = (int64_t)t;
-    }
-}
-
-__global__ void __launch_bounds__(1024)
-k_scan_top(int64_t *__restrict__ bsum, int64_t nb, int64_t base, const int32_t *__restrict__ stop) {
-    __shared__ int64_t sw[1024 / kWave];
-    if (*stop) return;
-    const int64_t per = (nb + 1023) / 1024;
-    const int64_t b0 = (int64_t)threadIdx.x * per;
-    int64_t s = 0;
-    for (int64_t i = b0; i < b0 + per && i < nb; ++i) s += bsum[i];
-    int64_t total;
-    int64_t run = base + block_exclusive_sum<1024 / kWave>(s, sw, &total);
-    for (int64_t i = b0; i < b0 + per && i < nb; ++i) {
-        const int64_t v = bsum[i];
-        bsum[i] = run;
-        run += v;
-    }
-}
-
-__global__ void __launch_bounds__(kScanT)
-k_scan_down(const uint32_t *__restrict__ in, int64_t n, const int64_t *__restrict__ bsum,
-            int64_t *__restrict__ out, const int32_t *__restrict__ stop) {
-    __shared__ int64_t sw[kScanT / kWave];
-    if (*stop) return;
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
-    // thread t scans the kScanPer consecutive entries [base + t*kScanPer, ...)
-    const int64_t i0 = base + (int64_t)threadIdx.x * kScanPer;
-    uint32_t v[kScanPer];
-    int64_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        v[k] = i0 + k < n ? in[i0 + k] : 0u;
-        s += v[k];
-    }
-    int64_t total;
-    int64_t run = bsum[blockIdx.x] + block_exclusive_sum<kScanT / kWave>(s, sw, &total);
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        if (i0 + k < n) out[i0 + k] = run;
-        run += v[k];
-    }
-}
-
-// Lanes of this wave whose 8-bit digit equals this lane's, among `valid`.
-__device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool valid) {
-    unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const unsigned long long bb = __ballot(bit);
-        peers &= bit ? bb : ~bb;
-    }
-    return peers;
-}
-
-// Stable scatter of one pass.  FIRST: source payload is the position itself.
-template <bool FIRST>
-__global__ void __launch_bounds__(kSortTileT)
-k_hs_scatter(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-             const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles,
-             const uint32_t *__restrict__ ksrc, const uint32_t *__restrict__ psrc, int shift,
-             const int64_t *__restrict__ scan, uint32_t *__restrict__ kdst, uint32_t *__restrict__ pdst,
-             const int32_t *__restrict__ stop) {
-    constexpr int NW = kSortTileT / kWave;   // 4 waves, 512 elements each
-    constexpr int CH = kHubTile / NW / kWave;   // 8 chunks of 64 per wave
-    __shared__ uint32_t whist[NW][256];
-    __shared__ int64_t wbase[NW][256];
-    if (*stop) return;
-    const int lane = lane_id();
-    const int w = (int)(threadIdx.x / kWave);
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
-        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
-        for (int i = threadIdx.x; i < NW * 256; i += kSortTileT) (&whist[0][0])[i] = 0;
-        uint32_t kv[CH], pv[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = w * (CH * kWave) + c * kWave + lane;
-            const int jc = j < x.len ? j : 0;
-            kv[c] = ksrc[x.c0 + jc];
-            pv[c] = FIRST ? (uint32_t)(x.tl * kHubTile + j) : psrc[x.c0 + jc];
-        }
-        __syncthreads();
-        unsigned long long peers[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = w * (CH * kWave) + c * kWave + lane;
-            const bool valid = j < x.len;
-            const uint32_t d = (kv[c] >> shift) & 0xffu;
-            peers[c] = digit_peers(d, valid);
-            if (valid && (peers[c] & lt) == 0) whist[w][d] += (uint32_t)__popcll(peers[c]);   // lowest peer
-        }
-        __syncthreads();
-        {
-            const int d = (int)threadIdx.x;   // 256 threads = 256 digits
-            int64_t b = scan[x.hbase + (int64_t)d * x.nth + x.tl];
-#pragma unroll
-            for (int ww = 0; ww < NW; ++ww) {
-                wbase[ww][d] = b;
-                b += whist[ww][d];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = w * (CH * kWave) + c * kWave + lane;
-            const bool valid = j < x.len;
-            const uint32_t d = (kv[c] >> shift) & 0xffu;
-            const int64_t base = wbase[w][d];
-            if (valid) {
-                const int64_t dst = base + __popcll(peers[c] & lt);
-                kdst[dst] = kv[c];
-                pdst[dst] = pv[c];
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (valid && (peers[c] & lt) == 0) wbase[w][d] = base + __popcll(peers[c]);
-            __builtin_amdgcn_wave_barrier();
-        }
-        __syncthreads();
-    }
-}
-
-// Run lengths of the sorted hub rows.  Thread t of a tile owns the
-// kSortEPT consecutive slots [t*8, t*8+8); a block max-scan of run starts
-// gives each run end its start; a run that began in an earlier tile finds
-// its start by binary search over the hub's sorted keys.  One atomicMax of
-// the tile's best (count << 32 | last) per tile.
-__global__ void __launch_bounds__(kSortTileT)
-k_hs_runs(const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-          const int32_t *__restrict__ hub_ntiles, const int64_t *__restrict__ hoff, int64_t tbase, int64_t ntiles,
-          const uint32_t *__restrict__ keys, const uint32_t *__restrict__ pos,
-          unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop) {
-    constexpr int NW = kSortTileT / kWave;
-    __shared__ int64_t sw[NW];
-    __shared__ unsigned long long sred64[NW];
-    if (*stop) return;
-    for (int64_t t = tbase + blockIdx.x; t < ntiles; t += gridDim.x) {
-        const HubTile x = hub_tile_at(t, tile_hub, hub_tile0, hub_ntiles, hoff);
-        const int64_t hb = hoff[x.h], he = hoff[x.h + 1];
-        const int j0 = (int)threadIdx.x * kSortEPT;
-        uint32_t kv[kSortEPT + 1];
-        int64_t last_start = -1;   // hub-local index of the last run start in my slots
-#pragma unroll
-        for (int k = 0; k <= kSortEPT; ++k) {
-            const int64_t g = x.c0 + j0 + k;
-            kv[k] = g < he ? keys[g] : 0xffffffffu;
-        }
-        const uint32_t kbefore = (x.c0 + j0 > hb) ? keys[x.c0 + j0 - 1] : 0xffffffffu;
-#pragma unroll
-        for (int k = 0; k < kSortEPT; ++k) {
-            const int j = j0 + k;
-            const uint32_t kp = k == 0 ? kbefore : kv[k - 1];
-            const bool start = j < x.len && (x.c0 + j == hb || kp != kv[k]);
-            if (start) last_start = x.c0 + j - hb;
-        }
-        // exclusive max-scan of last_start over threads (wave scan + waves)
-        int64_t inc = last_start;
-        const int lane = lane_id();
-        const int w = (int)(threadIdx.x / kWave);
-#pragma unroll
-        for (int s = 1; s < kWave; s <<= 1) {
-            const int64_t y = __shfl_up(inc, s);
-            if (lane >= s) inc = inc > y ? inc : y;
-        }
-        if (lane == kWave - 1) sw[w] = inc;
-        __syncthreads();
-        int64_t before = -1;
-#pragma unroll
-        for (int i = 0; i < NW; ++i)
-            if (i < w) before = before > sw[i] ? before : sw[i];
-        __syncthreads();
-        int64_t ex = __shfl_up(inc, 1);
-        if (lane == 0) ex = -1;
-        int64_t cur = ex > before ? ex : before;   // start of the run open at my first slot
-        unsigned long long best = 0;
-#pragma unroll
-        for (int k = 0; k < kSortEPT; ++k) {
-            const int j = j0 + k;
-            if (j >= x.len) break;
-            const uint32_t kp = k == 0 ? kbefore : kv[k - 1];
-            const int64_t gl = x.c0 + j - hb;   // hub-local index
-            if (gl == 0 || kp != kv[k]) cur = gl;
-            const bool end = (x.c0 + j + 1 == he) || kv[k + 1] != kv[k];
-            if (end) {
-                int64_t st = cur;
-                if (st < 0) {   // the run began in an earlier tile
-                    int64_t lo = 0, hi = x.c0 - hb;
+        for (int k = 0; k < EPT; ++k) {
+            const int q = k * T + tid;
+            run_end[k] = false;
+            if (q < len) {
+                key[k] = kf[q];
+                if (q == len - 1 || kf[q + 1] != key[k]) {
+                    int lo = 0, hi = q;
                     while (lo < hi) {
-                        const int64_t mid = (lo + hi) >> 1;
-                        if (keys[hb + mid] < kv[k]) lo = mid + 1;
+                        const int mid = (lo + hi) >> 1;
+                        if (kf[mid] < key[k]) lo = mid + 1;
                         else hi = mid;
                     }
-                    st = lo;
+                    run_end[k] = true;
+                    cnt[k] = (uint32_t)(q - lo + 1);
+                    last[k] = it.row0 + (uint32_t)pf[q];   // stable: the run's largest position
+                    part[k] = hub_partition(key[k], it.P);
+                    atomicAdd(&hist[part[k] + 1], 1u);
                 }
-                const unsigned long long val = ((unsigned long long)(gl - st + 1) << 32) | pos[x.c0 + j];
-                best = val > best ? val : best;
             }
         }
-        best = block_max<NW>(best, sred64);
-        if (threadIdx.x == 0 && best) atomicMax(&hub_best[x.h], best);
         __syncthreads();
+        // hist[p] := first slot of partition p (inclusive scan of the counts
+        // at p + 1), which are also the tile's partition offsets
+        {
+            const uint32_t np = it.P + 1;
+            const uint32_t per = (np + T - 1) / T;
+            const uint32_t p0 = (uint32_t)tid * per;
+            int run = 0;
+            for (uint32_t p = p0; p < p0 + per && p < np; ++p) run += (int)hist[p];
+            int acc = block_exclusive_scan<T / kWave>(run, sscan);   // barriers inside
+            for (uint32_t p = p0; p < p0 + per && p < np; ++p) {
+                acc += (int)hist[p];
+                hist[p] = (uint32_t)acc;
+                tile_idx[it.idx + p] = acc;
+            }
+        }
+        __syncthreads();
+        const int64_t ob = it.t * (int64_t)kHubTile;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+            if (run_end[k]) {
+                const uint32_t q = atomicAdd(&hist[part[k]], 1u);
+                out_key[ob + q] = key[k];
+                out_cnt[ob + q] = cnt[k];
+                out_last[ob + q] = last[k];
+            }
+        __syncthreads();
+        for (uint32_t p = (uint32_t)tid; p <= it.P; p += T) hist[p] = 0;
+    }
+};
+
+template <int T, typename E>
+__global__ void __launch_bounds__(T)
+k_hs_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+          const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
+          const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
+          const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles, uint32_t *__restrict__ out_key,
+          uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last, int32_t *__restrict__ tile_idx,
+          const int32_t *__restrict__ stop, int passes) {
+    constexpr int EPT = kHubTile / T;
+    __shared__ uint32_t skey[2 * kHubTile];
+    __shared__ uint16_t spos[2 * kHubTile];
+    __shared__ uint32_t whist[(T / kWave) * 256];
+    __shared__ uint32_t hist[kHubMaxParts + 1];
+    __shared__ int sscan[T / kWave];
+    if (*stop) return;
+    for (int p = threadIdx.x; p <= kHubMaxParts; p += T) hist[p] = 0;
+    HubSortTilePolicy<T, E> pol{off,  tile_hub, hub_tile0, hub_vid, hub_nparts, tile_idx_off, ntiles,  skey,
+                                spos, whist,    hist,      sscan,   out_key,    out_cnt,      out_last, tile_idx,
+                                passes};
+    pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+}
+
+// Block inclusive prefix sum of x[0, len) into y[0, len) (len <= T * EPT):
+// thread t sums its EPT consecutive elements, one block scan of the sums.
+template <int T, int EPT>
+__device__ __forceinline__ void block_prefix_sum(const uint32_t *x, uint32_t *y, int len, int *sscan) {
+    const int tid = (int)threadIdx.x;
+    const int b = tid * EPT;
+    uint32_t v[EPT];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        v[k] = b + k < len ? x[b + k] : 0u;
+        sum += (int)v[k];
+    }
+    uint32_t acc = (uint32_t)block_exclusive_scan<T / kWave>(sum, sscan);   // barriers inside
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        acc += v[k];
+        if (b + k < len) y[b + k] = acc;
+    }
+}
+
+// Merge of one round of staged entries: keys in skey[0, len) (first half),
+// their counts and last positions in scnt / slast at the same slots.  Sorts
+// by label (entry slot as payload), reduces each run to (label, summed
+// count, last entry's position) and writes the runs back, in label order, to
+// slots [0, runs).  Returns the number of runs (uniform).
+template <int T, int N>
+__device__ int hs_merge_round(uint32_t *skey, uint16_t *spos, uint32_t *whist, uint32_t *scnt, uint32_t *slast,
+                              int len, int passes, int *sscan, int *snruns) {
+    constexpr int EPT = N / T;
+    const int tid = (int)threadIdx.x;
+    for (int i = tid; i < len; i += T) spos[i] = (uint16_t)i;
+    // lds_radix_sort's first barrier orders these writes before its reads
+    const int src = lds_radix_sort<T, N>(skey, spos, whist, len, passes);
+    const uint32_t *kf = skey + src * N;
+    const uint16_t *pf = spos + src * N;
+    uint32_t *csort = skey + (src ^ 1) * N;   // free half: counts in sorted order, then their prefix sums
+    for (int q = tid; q < len; q += T) csort[q] = scnt[pf[q]];
+    __syncthreads();
+    block_prefix_sum<T, EPT>(csort, csort, len, sscan);
+    __syncthreads();
+    // each thread: the run ends among its EPT consecutive sorted slots
+    uint32_t rk[EPT], rc[EPT], rl[EPT];
+    bool end[EPT];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int q = tid * EPT + k;
+        end[k] = false;
+        if (q < len && (q == len - 1 || kf[q + 1] != kf[q])) {
+            const uint32_t key = kf[q];
+            int lo = 0, hi = q;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (kf[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            end[k] = true;
+            rk[k] = key;
+            rc[k] = csort[q] - (lo ? csort[lo - 1] : 0u);
+            rl[k] = slast[pf[q]];
+            ++mine;
+        }
+    }
+    int r = block_exclusive_scan<T / kWave>(mine, sscan);   // barriers inside: every read above is done
+    if (tid == T - 1) *snruns = r + mine;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k)
+        if (end[k]) {
+            skey[r] = rk[k];
+            scnt[r] = rc[k];
+            slast[r] = rl[k];
+            ++r;
+        }
+    __syncthreads();
+    return *snruns;
+}
+
+// One block per merge item (hub, partition), grid-stride over the items.
+template <int T>
+__global__ void __launch_bounds__(T)
+k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
+           int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
+           const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
+           const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_cnt,
+           const uint32_t *__restrict__ out_last, unsigned long long *__restrict__ hub_best,
+           const int32_t *__restrict__ stop, int32_t *__restrict__ err, int passes) {
+    constexpr int N = kHsMergeCap;
+    __shared__ uint32_t skey[2 * N];
+    __shared__ uint16_t spos[2 * N];
+    __shared__ uint32_t scnt[N], slast[N];
+    __shared__ uint32_t whist[(T / kWave) * 256];
+    __shared__ int32_t slo[T], sstart[T];
+    __shared__ int sscan[T / kWave];
+    __shared__ int stotal, snruns;
+    __shared__ unsigned long long sred64[T / kWave];
+    if (*stop) return;
+    const int tid = (int)threadIdx.x;
+    for (int64_t it = ibase + blockIdx.x; it < ibase + nitems; it += gridDim.x) {
+        const int h = item_hub[it];
+        const int p = item_part[it];
+        const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        int held = 0;   // staged slots: reduced runs first, then new entries (uniform)
+        bool overflow = false;
+        for (int64_t c0 = t0; c0 < t0 + nt && !overflow; c0 += T) {
+            // this chunk of T tiles: slice bounds one tile per thread, a scan
+            // of the slice lengths maps a flat index back to its tile
+            __syncthreads();
+            const int64_t t = c0 + tid;
+            int lo = 0, len = 0;
+            if (t < t0 + nt) {
+                const int64_t ib = tile_idx_off[t];
+                lo = tile_idx[ib + p];
+                len = tile_idx[ib + p + 1] - lo;
+            }
+            const int start = block_exclusive_scan<T / kWave>(len, sscan);   // barriers inside
+            slo[tid] = lo;
+            sstart[tid] = start;
+            if (tid == T - 1) stotal = start + len;
+            __syncthreads();
+            const int total = stotal;
+            const int nc = (int)min((int64_t)T, t0 + nt - c0);
+            for (int i0 = 0; i0 < total;) {
+                if (held == N) {   // full: merge what is staged
+                    held = hs_merge_round<T, N>(skey, spos, whist, scnt, slast, held, passes, sscan, &snruns);
+                    if (held == N) {   // N distinct labels in one partition: no room to progress
+                        overflow = true;
+                        break;
+                    }
+                }
+                const int take = min(N - held, total - i0);
+                for (int j = tid; j < take; j += T) {
+                    const int i = i0 + j;
+                    int a = 0, b = nc;   // last tile with start <= i
+                    while (b - a > 1) {
+                        const int mid = (a + b) >> 1;
+                        if (sstart[mid] <= i) a = mid;
+                        else b = mid;
+                    }
+                    const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - sstart[a]);
+                    skey[held + j] = out_key[e];
+                    scnt[held + j] = out_cnt[e];
+                    slast[held + j] = out_last[e];
+                }
+                held += take;
+                i0 += take;
+                __syncthreads();
+            }
+        }
+        if (overflow) {
+            if (tid == 0) atomicOr(err, 16);
+            continue;
+        }
+        if (held > 0) held = hs_merge_round<T, N>(skey, spos, whist, scnt, slast, held, passes, sscan, &snruns);
+        unsigned long long best = 0;
+        for (int r = tid; r < held; r += T) {
+            const unsigned long long pk = ((unsigned long long)scnt[r] << 32) | slast[r];
+            best = pk > best ? pk : best;
+        }
+        best = block_max<T / kWave>(best, sred64);
+        if (tid == 0 && best) atomicMax(&hub_best[h], best);
     }
 }
 
@@ -1011,37 +990,19 @@ static int salloc(galp_graph *g, T **p, int64_t count) {
 }
 
 void sort_free(galp_graph *g) {
-    SortState *S = g->sort;
-    if (!S) return;
-    void *ptrs[] = {S->d_hoff, S->d_kA, S->d_kB, S->d_pA, S->d_pB, S->d_hist, S->d_scan, S->d_bsum};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    delete S;
+    delete g->sort;
     g->sort = nullptr;
 }
 
-int sort_setup(galp_graph *g, const std::vector<int64_t> &hub_deg) {
+// The radix passes labels < n need; the hub path's buffers belong to the hub
+// plan both methods share (api.hip).
+int sort_setup(galp_graph *g) {
     if (g->sort) return GALP_OK;
     SortState *S = new (std::nothrow) SortState();
     if (!S) return fail(g->ctx, GALP_ENOMEM, "host allocation failed");
     g->sort = S;
     const int bits = g->n > 1 ? 64 - __builtin_clzll((unsigned long long)(g->n - 1)) : 1;
     S->passes = (bits + 7) / 8;
-    std::vector<int64_t> hoff(hub_deg.size() + 1, 0);
-    for (size_t h = 0; h < hub_deg.size(); ++h) hoff[h + 1] = hoff[h] + hub_deg[h];
-    S->eh = hoff.back();
-    S->hoff_host = hoff;
-    if (!g->nhubs) return GALP_OK;
-    S->nscan = g->nhub_tiles * 256;
-    S->nbsum = (S->nscan + kScanChunk - 1) / kScanChunk;
-    int rc;
-    if ((rc = salloc(g, &S->d_hoff, (int64_t)hoff.size())) || (rc = salloc(g, &S->d_kA, S->eh)) ||
-        (rc = salloc(g, &S->d_kB, S->eh)) || (rc = salloc(g, &S->d_pA, S->eh)) || (rc = salloc(g, &S->d_pB, S->eh)) ||
-        (rc = salloc(g, &S->d_hist, S->nscan)) || (rc = salloc(g, &S->d_scan, S->nscan)) ||
-        (rc = salloc(g, &S->d_bsum, S->nbsum)))
-        return rc;
-    if (hipMemcpy(S->d_hoff, hoff.data(), 8 * hoff.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return fail(g->ctx, GALP_EHIP, "hub offsets upload failed");
     return GALP_OK;
 }
 
@@ -1101,46 +1062,24 @@ static void sort_reg(galp_graph *g, const IterView &w, int bid, const int32_t *p
     prof_end(g, &pl);
 }
 
-// Hubs [h0, h1) with tiles [t0, t1): the histogram, scan and scatter of every
-// pass cover just those tiles; the scan starts at the hubs' first compact
-// slot, so the output lands at the same places as in a whole-graph pass.
+// Hubs [h0, h1) of a view, with their tiles [t0, t1) and merge items
+// [i0, i1): tile sorts, partition merges, winners.  tile_per_cu and
+// merge_per_cu size the grids (the autotuner's options).
 template <typename E>
-static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next, int tiles_per_cu = 8) {
-    SortState *S = g->sort;
-    const int64_t t0 = w.t0, t1 = w.t1, nt = t1 - t0, nh = w.h1 - w.h0;
-    const int gt = grid_for(g, nt, tiles_per_cu);
-    const int64_t nscan = nt * 256;
-    const int nbs = (int)((nscan + kScanChunk - 1) / kScanChunk);
-    uint32_t *hist = S->d_hist + t0 * 256;
-    int64_t *scan = S->d_scan + t0 * 256;
+static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next, int tile_per_cu = 3,
+                     int merge_per_cu = 1) {
+    const int64_t nt = w.t1 - w.t0, ni = w.i1 - w.i0, nh = w.h1 - w.h0;
     double hub_edges = 0.0;
     if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
     PendingLaunch pl;
     prof_begin(g, "hub_sort", 8.0 * hub_edges, &pl);
-    k_hs_gather_hist<E><<<gt, kSortTileT, 0, g->stream>>>((const E *)w.off, w.nbr, prev, g->d_hub_vid,
-                                                          g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff,
-                                                          t0, t1, S->d_kA, S->d_hist, g->d_stop);
-    uint32_t *ks = S->d_kA, *ps = S->d_pA, *kd = S->d_kB, *pd = S->d_pB;
-    for (int p = 0; p < S->passes; ++p) {
-        if (p > 0)
-            k_hs_hist<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, t0,
-                                                        t1, ks, 8 * p, S->d_hist, g->d_stop);
-        k_scan_reduce<<<nbs, kScanT, 0, g->stream>>>(hist, nscan, S->d_bsum, g->d_stop);
-        k_scan_top<<<1, 1024, 0, g->stream>>>(S->d_bsum, nbs, S->hoff_host[(size_t)w.h0], g->d_stop);
-        k_scan_down<<<nbs, kScanT, 0, g->stream>>>(hist, nscan, S->d_bsum, scan, g->d_stop);
-        if (p == 0)
-            k_hs_scatter<true><<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles,
-                                                                 S->d_hoff, t0, t1, ks, nullptr, 0, S->d_scan, kd, pd,
-                                                                 g->d_stop);
-        else
-            k_hs_scatter<false><<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles,
-                                                                  S->d_hoff, t0, t1, ks, ps, 8 * p, S->d_scan, kd,
-                                                                  pd, g->d_stop);
-        std::swap(ks, kd);
-        std::swap(ps, pd);
-    }
-    k_hs_runs<<<gt, kSortTileT, 0, g->stream>>>(g->d_tile_hub, g->d_hub_tile0, g->d_hub_ntiles, S->d_hoff, t0, t1,
-                                                 ks, ps, g->d_hub_best, g->d_stop);
+    k_hs_tile<kHsTileT, E><<<grid_for(g, nt, tile_per_cu), kHsTileT, 0, g->stream>>>(
+        (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
+        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+        g->sort->passes);
+    k_hs_merge<kHsMergeT><<<grid_for(g, ni, merge_per_cu), kHsMergeT, 0, g->stream>>>(
+        g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off, g->d_tile_idx,
+        g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, g->sort->passes);
     k_hs_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
         (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop);
     prof_end(g, &pl);
@@ -1150,14 +1089,15 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
 // The sort method's configuration of each degree class (g->tune[b]): option
 // 0 is the default, 1 and 2 the alternatives the autotuner times --
 // workgroup size x keys per thread, the radix sort or the bitonic network,
-// blocks per CU; for lane groups the grid; for hubs the tiles per CU of the
-// histogram and scatter grids.  Every option computes the same labels.
+// blocks per CU; for lane groups the grid; for hubs the blocks per CU of the
+// tile grid (3 or 2) and of the merge grid (1 or 2).  Every option computes
+// the same labels.
 template <typename E>
 static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
     const int c = g->tune[b];
     switch (b) {
     case B_HUB:
-        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 4 : c == 2 ? 16 : 8);
+        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 2 : 3, c == 2 ? 2 : 1);
         break;
     case B_B4K:
         if (c == 1) sort_reg<4, 16, E>(g, w, b, prev, next, 2);
