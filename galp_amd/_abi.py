@@ -130,6 +130,7 @@ def lib():
         "galp_mlp_num_params": (i64, [i32]),
         "galp_mlp_init": (ctypes.c_int, [i32, ctypes.c_uint64, vp]),
         "galp_mlp_predict": (ctypes.c_int, [vp, i32, vp, vp, i64, vp]),
+        "galp_mlp_loss_grad_noalloc": (ctypes.c_int, [vp, i32, vp, vp, vp, i64, vp, vp, vp]),
         "galp_mlp_loss_grad": (ctypes.c_int, [vp, i32, vp, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_float)]),
         "galp_mlp_train": (ctypes.c_int, [vp, i32, vp, vp, vp, i64, i32, i32, ctypes.c_float, ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_float)]),

@@ -610,6 +610,10 @@ GALP_API int galp_open(int device, galp_ctx **out) {
         return GALP_EHIP;
     }
     ctx->device = device;
+    if (mlp_scratch_setup(ctx, device) != GALP_OK) {
+        delete ctx;
+        return GALP_EHIP;
+    }
     *out = ctx;
     return GALP_OK;
 }

@@ -328,6 +328,10 @@ int launch_ooc_iteration(galp_graph *g);
 // Streams every batch through learn_features_view once (learned dispatch set-up).
 int ooc_features(galp_graph *g);
 
+// Looks up the allocation-free entry point's scratch array on `device`
+// (current) once per device (mlp.hip).
+int mlp_scratch_setup(galp_ctx *ctx, int device);
+
 // Drops the captured iteration loop after a change to what it launches.
 inline void invalidate_captured(galp_graph *g) {
     if (g->gexec) (void)hipGraphExecDestroy(g->gexec);

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -332,6 +333,42 @@ struct DevBuf {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Allocation-free loss and gradient (galp_mlp_loss_grad_noalloc): the
+// per-block partial gradients and losses of k_mlp_grad live in this
+// file-scope device array instead of memory allocated per call.  One copy per
+// device, held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is a stand-in, not a device address):
+// mlp_scratch_setup() looks up each device's address once, with that device
+// current, and the entry point passes that address.
+constexpr int64_t kNoAllocMaxSamples = 32768;                    // per call
+constexpr int64_t kNoAllocMaxBlocks = kNoAllocMaxSamples / 64;   // k_mlp_grad blocks of 64 samples
+constexpr int64_t kNoAllocScratchFloats = kNoAllocMaxBlocks * mlp_num_params(32) + kNoAllocMaxBlocks;
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];   // 2.56 MB
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves the device address of `scratch` on `device` (which must be
+// current), once per device.  Called by galp_open.
+int mlp_scratch_setup(galp_ctx *ctx, int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(ctx, GALP_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return GALP_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(ctx, GALP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (float *)p;
+    return GALP_OK;
+}
+
+static float *scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace galp
 
 using namespace galp;
@@ -424,6 +461,28 @@ GALP_API int galp_mlp_loss_grad(galp_ctx *ctx, int32_t hidden, const float *para
     GALP_CK(hipMemcpyAsync(loss, dl.p, 4, hipMemcpyDeviceToHost, st));
     GALP_CK(hipStreamSynchronize(st));
     (void)hipStreamDestroy(st);
+    return GALP_OK;
+}
+
+GALP_API int galp_mlp_loss_grad_noalloc(galp_ctx *ctx, int32_t hidden, const float *d_params, const float *d_x,
+                                        const float *d_y, int64_t n, float *d_grad, float *d_loss, void *stream) {
+    if (!ctx || !d_params || !d_x || !d_y || !d_grad || !d_loss || n <= 0) return GALP_EINVAL;
+    if (hidden != 8 && hidden != 16 && hidden != 32) return fail(ctx, GALP_EINVAL, "hidden must be 8, 16 or 32");
+    if (n > kNoAllocMaxSamples)
+        return fail(ctx, GALP_EINVAL, "galp_mlp_loss_grad_noalloc: n = " + std::to_string(n) +
+                                          " samples exceeds the scratch limit of " +
+                                          std::to_string(kNoAllocMaxSamples) + " per call");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != ctx->device)
+        return fail(ctx, GALP_EINVAL, "galp_mlp_loss_grad_noalloc: the current device is not the context's");
+    float *sp = scratch_of(ctx->device);
+    if (!sp) return fail(ctx, GALP_EHIP, "galp_mlp_loss_grad_noalloc: scratch not set up on this device");
+    float *pgrad = sp;                                            // ceil(n / 64) * P floats
+    float *ploss = sp + kNoAllocMaxBlocks * mlp_num_params(32);   // ceil(n / 64) floats
+    // every slot the kernels read is written by them in this call: no clearing needed
+    if (mlp_grad_dev((hipStream_t)stream, hidden, d_params, d_x, d_y, n, pgrad, ploss, d_grad, d_loss) != GALP_OK)
+        return fail(ctx, GALP_EHIP, std::string("galp_mlp_loss_grad_noalloc: launch failed: ") +
+                                        hipGetErrorString(hipGetLastError()));
     return GALP_OK;
 }
 

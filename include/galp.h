@@ -214,6 +214,19 @@ GALP_API int galp_mlp_predict(galp_ctx *ctx, int32_t hidden, const float *params
 GALP_API int galp_mlp_loss_grad(galp_ctx *ctx, int32_t hidden, const float *params,
                                 const float *x, const float *y, int64_t n,
                                 float *grad, float *loss);
+/* galp_mlp_loss_grad on device-resident data, allocating nothing.  No
+ * reference counterpart: it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  d_params, d_x[n][4], d_y[n], d_grad[P] and d_loss[1]
+ * are device pointers on the context's device, which must be current;
+ * the kernels are queued on `stream` (a hipStream_t; NULL = the null
+ * stream) and the call returns without waiting for them.  The per-block
+ * partial gradients live in a fixed device array of the library (2.56 MB per
+ * device, held until the process ends), so n is limited to 32768 samples
+ * (GALP_EINVAL beyond), and calls that overlap on one device race on it. */
+GALP_API int galp_mlp_loss_grad_noalloc(galp_ctx *ctx, int32_t hidden, const float *d_params,
+                                        const float *d_x, const float *d_y, int64_t n,
+                                        float *d_grad, float *d_loss, void *stream);
 /* Adam over `epochs` passes of minibatches of `batch` samples (seeded
  * shuffle); params are updated in place; final_loss = loss over all n. */
 GALP_API int galp_mlp_train(galp_ctx *ctx, int32_t hidden, float *params,
