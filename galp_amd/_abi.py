@@ -141,6 +141,7 @@ def lib():
         "galp_graph_dispatch_stats": (ctypes.c_int, [vp, vp, i32, ctypes.POINTER(i32)]),
         "galp_graph_set_batch": (ctypes.c_int, [vp, i64, vp]),
         "galp_graph_autotune": (ctypes.c_int, [vp, i32, ctypes.c_char_p, vp]),
+        "galp_graph_set_learned_iterations": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "galp_graph_set_tuning": (ctypes.c_int, [vp, vp]),
         "galp_batch_iterations": (ctypes.c_int, [vp, vp]),
     }
@@ -370,7 +371,12 @@ class Graph:
         self.ctx.check(lib().galp_graph_autotune(self.h, reps, os.fsencode(cache_path) if cache_path else None,
                                                  out.ctypes.data))
         self.tuned_streams = int(out[12])   # 1 or 3 streams per iteration (0: not timed)
+        # learned dispatch: iterations the judge left to the network
+        self.learned_iterations = int(np.uint32(out[13])) | (int(np.uint32(out[14])) << 32)
         return out[:12]
+
+    def set_learned_iterations(self, mask):
+        self.ctx.check(lib().galp_graph_set_learned_iterations(self.h, ctypes.c_uint64(mask & (2 ** 64 - 1))))
 
     def set_tuning(self, options):
         o = np.ascontiguousarray(options, np.int32)

@@ -118,25 +118,6 @@ static int64_t hub_min_of(const galp_opts &o) {
     return kBucketHi[B_B4K];
 }
 
-template <class T>
-static int dev_alloc(galp_graph *g, T **p, int64_t count) {
-    *p = nullptr;
-    if (count <= 0) count = 1;
-    size_t bytes = sizeof(T) * (size_t)count;
-    if (g->opts.mem_budget > 0 && g->device_bytes + (int64_t)bytes > g->opts.mem_budget)
-        return fail(g->ctx, GALP_ENOMEM,
-                    "device memory budget exceeded: need " + std::to_string(g->device_bytes + (int64_t)bytes) +
-                        " bytes, budget " + std::to_string(g->opts.mem_budget));
-    hipError_t e = hipMalloc((void **)p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return fail(g->ctx, GALP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
-                                             hipGetErrorString(e));
-    }
-    g->device_bytes += (int64_t)bytes;
-    return GALP_OK;
-}
 
 static void dev_free_all(galp_graph *g) {
     learn_free(g);
@@ -421,29 +402,29 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_hub_vid, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_nparts, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_best, g->nhubs));
-        GALP_TRY(dev_alloc(g, &g->d_hub_tile0, g->nhubs));
-        GALP_TRY(dev_alloc(g, &g->d_hub_ntiles, g->nhubs));
-        GALP_TRY(dev_alloc(g, &g->d_tile_hub, g->nhub_tiles));
         auto up = [&](void *dst, const void *src, size_t bytes) {
             return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
         };
         GALP_HIP(ctx, up(g->d_hub_vid, hub_vertices.data(), 4 * hub_vertices.size()));
         GALP_HIP(ctx, up(g->d_hub_nparts, hub_nparts.data(), 4 * hub_nparts.size()));
+        // row-order tiles of kHubTile edges
+        GALP_TRY(dev_alloc(g, &g->d_hub_tile0, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_hub_ntiles, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_tile_hub, g->nhub_tiles));
+        GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
+        GALP_TRY(dev_alloc(g, &g->d_tile_idx, ntile_idx));
+        GALP_TRY(dev_alloc(g, &g->d_out_key, g->nhub_tiles * kHubTile));
+        GALP_TRY(dev_alloc(g, &g->d_out_cnt, g->nhub_tiles * kHubTile));
+        GALP_TRY(dev_alloc(g, &g->d_out_last, g->nhub_tiles * kHubTile));
         GALP_HIP(ctx, up(g->d_hub_tile0, hub_tile0.data(), 4 * hub_tile0.size()));
         GALP_HIP(ctx, up(g->d_hub_ntiles, hub_ntiles.data(), 4 * hub_ntiles.size()));
         GALP_HIP(ctx, up(g->d_tile_hub, tile_hub.data(), 4 * tile_hub.size()));
-        {   // tile outputs and merge items of both methods' hub paths
-            GALP_TRY(dev_alloc(g, &g->d_item_hub, g->nhub_items));
-            GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
-            GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
-            GALP_TRY(dev_alloc(g, &g->d_tile_idx, ntile_idx));
-            GALP_TRY(dev_alloc(g, &g->d_out_key, g->nhub_tiles * kHubTile));
-            GALP_TRY(dev_alloc(g, &g->d_out_cnt, g->nhub_tiles * kHubTile));
-            GALP_TRY(dev_alloc(g, &g->d_out_last, g->nhub_tiles * kHubTile));
-            GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
-            GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
-            GALP_HIP(ctx, up(g->d_tile_idx_off, tile_idx_off.data(), 8 * tile_idx_off.size()));
-        }
+        GALP_HIP(ctx, up(g->d_tile_idx_off, tile_idx_off.data(), 8 * tile_idx_off.size()));
+        // merge items (hub, label partition) of both methods' hub paths
+        GALP_TRY(dev_alloc(g, &g->d_item_hub, g->nhub_items));
+        GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
+        GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
+        GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
     }
     if (ooc) {
@@ -503,6 +484,7 @@ int side_join(galp_graph *g, int k) {
 }
 
 static int launch_iteration(galp_graph *g) {
+    if (g->batch) GALP_TRY(launch_batch_lists(g));   // drops the graphs that stopped
     switch (g->opts.method) {
     case GALP_METHOD_HASH:
         if (g->opts.dispatch == GALP_DISPATCH_LEARNED) {
@@ -760,7 +742,7 @@ GALP_API int galp_lp_step(galp_graph *g, galp_step_stats *st) {
     if (rc == GALP_OK) {
         hipError_t e = hipMemcpyAsync(g->h_pinned, g->d_iter_counts + idx, 8, hipMemcpyDeviceToHost, g->stream);
         g->h_pinned[9] = 0;
-        if (e == hipSuccess && g->learn && g->opts.dispatch == GALP_DISPATCH_LEARNED)
+        if (e == hipSuccess && learned_now(g))
             e = hipMemcpyAsync(g->h_pinned + 9, g->learn->d_plist_n + kCtrOverflow, 4, hipMemcpyDeviceToHost,
                                g->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(g->h_pinned + 2, g->d_stop, 4, hipMemcpyDeviceToHost, g->stream);

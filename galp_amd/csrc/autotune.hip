@@ -21,6 +21,9 @@
 // choice, 1 or 3):
 //   <arch> <method> <class> <log2 vertices> <log2 edges> <option>
 // and are reused for a graph whose class has the same arch, method and sizes.
+// Under learned dispatch, in core, the tuner then judges the network against
+// the plain dispatcher iteration by iteration (the per-iteration mask,
+// galp_graph.learn_mask), which is not cached: it depends on the network.
 
 #include <algorithm>
 #include <cmath>
@@ -110,6 +113,14 @@ GALP_API int galp_graph_set_tuning(galp_graph *g, const int32_t *options) {
     return GALP_OK;
 }
 
+GALP_API int galp_graph_set_learned_iterations(galp_graph *g, uint64_t mask) {
+    if (!g) return GALP_EINVAL;
+    if (g->opts.dispatch != GALP_DISPATCH_LEARNED) return fail(g->ctx, GALP_ESTATE, "not a learned-dispatch graph");
+    g->learn_mask = mask;
+    invalidate_captured(g);
+    return GALP_OK;
+}
+
 GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_path, int32_t *chosen) {
     if (!g || reps < 1) return GALP_EINVAL;
     const bool hash = g->opts.method == GALP_METHOD_HASH;
@@ -191,6 +202,37 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
         }
         g->streams = (int8_t)mode;
     }
+    if (rc == GALP_OK && g->opts.dispatch == GALP_DISPATCH_LEARNED && !g->ooc) {
+        // the judge: iteration by iteration, routing by the network against
+        // routing by the degree classes alone (the plain dispatcher), on
+        // whole iterations (device time, all streams) of the same run -- the
+        // labels, hence every iteration's work, are the same either way.  An
+        // iteration keeps the network where it is not slower; iterations past
+        // the timed ones follow the last of them.
+        double t[2][kTuneIters];
+        for (int q = 0; q < 2; ++q)
+            for (int i = 0; i < kTuneIters; ++i) t[q][i] = 1e300;
+        int nit = kTuneIters;
+        for (int round = 0; round < reps && rc == GALP_OK; ++round)
+            for (int q = 0; q < 2 && rc == GALP_OK; ++q) {
+                g->learn_mask = q ? ~0ull : 0ull;   // q = 1: the network
+                rc = galp_lp_init(g);
+                for (int i = 0; i < kTuneIters && rc == GALP_OK; ++i) {
+                    galp_step_stats st{};
+                    rc = galp_lp_step(g, &st);
+                    if (rc == GALP_OK) t[q][i] = std::min(t[q][i], st.device_ms);
+                    if (rc == GALP_OK && st.stopped) {
+                        nit = std::min(nit, i + 1);
+                        break;
+                    }
+                }
+            }
+        uint64_t mask = 0;
+        for (int i = 0; i < nit; ++i)
+            if (t[1][i] <= t[0][i]) mask |= 1ull << i;
+        if (nit > 0 && ((mask >> (nit - 1)) & 1ull)) mask |= ~0ull << nit;
+        g->learn_mask = rc == GALP_OK ? mask : ~0ull;
+    }
     g->opts.dispatch = dispatch;
     g->opts.graph_capture = capture;
     invalidate_captured(g);
@@ -207,6 +249,10 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
         for (int b = 0; b < NUM_BUCKETS; ++b) chosen[b] = g->tune[b];
         chosen[kStreamsLine] = g->streams;
         for (int b = kStreamsLine + 1; b < 16; ++b) chosen[b] = 0;
+        if (g->opts.dispatch == GALP_DISPATCH_LEARNED) {   // the judge's mask
+            chosen[kStreamsLine + 1] = (int32_t)(uint32_t)g->learn_mask;
+            chosen[kStreamsLine + 2] = (int32_t)(uint32_t)(g->learn_mask >> 32);
+        }
     }
     return GALP_OK;
 }

@@ -11,6 +11,13 @@
 // stop flag when every graph has stopped.  The rule is the reference's
 // (lfht.cuh:97, dpp.cuh:88) applied per graph: stop after an iteration with
 // updated_k <= n_k * 1e-5.
+// Under the hash method with plain dispatch, the graphs that stopped also
+// leave the work: before every iteration k_batch_lists rebuilds, per degree
+// class below the hubs, the list of the running graphs' vertices (lengths
+// on the device), the LDS classes walk their lists, and the five lane-group
+// classes run as one launch over theirs (k_group_lists, lp_hash.hip), so a
+// frozen graph costs no gathers; the epilogue keeps its labels in both
+// buffers.
 
 #include <algorithm>
 #include <vector>
@@ -45,6 +52,46 @@ __global__ void k_batch_check(const E *__restrict__ off, const int32_t *__restri
                 atomicOr(bad, 1);
                 return;
             }
+    }
+}
+
+// The work lists: the members of classes G4..B4K (flat index over the
+// classes: class b holds [f0[b], f0[b+1])) whose graph is still running, each
+// appended to its class's list; appends are aggregated per wave and class.
+struct ClassSpans {
+    int64_t v0[NUM_BUCKETS];   // first id, or first index into vlist
+    int64_t f0[NUM_BUCKETS + 1];
+};
+
+__global__ void __launch_bounds__(256)
+k_batch_lists(ClassSpans cs, const int32_t *__restrict__ vlist, const int32_t *__restrict__ gid,
+              const int32_t *__restrict__ gstop, int32_t *const *__restrict__ alist, int64_t *__restrict__ alen,
+              const int32_t *__restrict__ stop) {
+    if (*stop) return;
+    const int64_t total = cs.f0[B_HUB];
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
+        const int64_t i = base + threadIdx.x;
+        int b = 0;
+        bool keep = false;
+        int32_t v = 0;
+        if (i < total) {
+            while (i >= cs.f0[b + 1]) ++b;
+            const int64_t k = cs.v0[b] + (i - cs.f0[b]);
+            v = vlist ? vlist[k] : (int32_t)k;
+            keep = gstop[gid[v]] == 0;
+        }
+        unsigned long long ball;
+        while ((ball = __ballot(keep)) != 0ull) {
+            const int leader = __ffsll((long long)ball) - 1;
+            const int bl = __builtin_amdgcn_readlane(b, leader);
+            const bool mine = keep && b == bl;
+            const unsigned long long m = __ballot(mine);
+            unsigned long long at = 0;
+            if (lane_id() == leader) at = atomicAdd((unsigned long long *)(alen + bl), (unsigned long long)__popcll(m));
+            at = __shfl(at, leader);
+            if (mine) alist[bl][at + __popcll(m & ((1ull << lane_id()) - 1ull))] = v;
+            keep = keep && !mine;
+        }
     }
 }
 
@@ -125,9 +172,11 @@ __global__ void k_batch_reset(int32_t *__restrict__ gstop, int32_t *__restrict__
 void batch_free(galp_graph *g) {
     BatchState *B = g->batch;
     if (!B) return;
-    void *ptrs[] = {B->d_gid, B->d_gn, B->d_gcount, B->d_gstop, B->d_giters, B->d_vbase};
+    void *ptrs[] = {B->d_gid, B->d_gn, B->d_gcount, B->d_gstop, B->d_giters, B->d_vbase, B->d_alen, B->d_alist_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (int b = 0; b < NUM_BUCKETS; ++b)
+        if (B->d_alist[b]) (void)hipFree(B->d_alist[b]);
     delete B;
     g->batch = nullptr;
 }
@@ -138,6 +187,26 @@ int batch_reset(galp_graph *g) {
     k_batch_reset<<<(int)std::min<int64_t>((B->ngraphs + 255) / 256, 1024), 256, 0, g->stream>>>(
         B->d_gstop, B->d_giters, B->d_gcount, B->ngraphs);
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "batch reset failed");
+}
+
+int launch_batch_lists(galp_graph *g) {
+    BatchState *B = g->batch;
+    if (!B || !B->lists) return GALP_OK;
+    ClassSpans cs{};
+    cs.f0[0] = 0;
+    for (int b = 0; b < NUM_BUCKETS; ++b) {
+        cs.v0[b] = g->bucket[b].v0;
+        cs.f0[b + 1] = cs.f0[b] + (b < B_HUB ? g->bucket[b].nv : 0);
+    }
+    int rc = dev_fill_u32(g, (uint32_t *)B->d_alen, 2 * NUM_BUCKETS, 0u);
+    if (rc != GALP_OK) return rc;
+    const int64_t total = cs.f0[B_HUB];
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256,
+                                                                 (int64_t)g->ctx->prop.multiProcessorCount * 8));
+    k_batch_lists<<<grid, 256, 0, g->stream>>>(cs, g->d_vlist, B->d_gid, B->d_gstop, B->d_alist_dev, B->d_alen,
+                                               g->d_stop);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("batch lists: ") + hipGetErrorString(e));
 }
 
 int launch_batch_end(galp_graph *g) {
@@ -216,6 +285,34 @@ GALP_API int galp_graph_set_batch(galp_graph *g, int64_t ngraphs, const int64_t 
     if (bad) {
         batch_free(g);
         return fail(g->ctx, GALP_EINVAL, "an edge joins two graphs of the batch");
+    }
+    if (g->opts.method == GALP_METHOD_HASH && g->opts.dispatch == GALP_DISPATCH_PLAIN) {
+        // work lists of the running graphs' vertices, one per class below the hubs
+        bool okl = hipMalloc((void **)&B->d_alen, 8 * NUM_BUCKETS) == hipSuccess &&
+                   hipMalloc((void **)&B->d_alist_dev, sizeof(int32_t *) * NUM_BUCKETS) == hipSuccess;
+        for (int b = 0; b < B_HUB && okl; ++b)
+            okl = hipMalloc((void **)&B->d_alist[b], 4 * (size_t)std::max<int64_t>(1, g->bucket[b].nv)) == hipSuccess;
+        if (okl)
+            okl = hipMemcpy(B->d_alist_dev, B->d_alist, sizeof(int32_t *) * NUM_BUCKETS, hipMemcpyHostToDevice) ==
+                  hipSuccess;
+        if (!okl) {
+            (void)hipGetLastError();
+            batch_free(g);
+            return fail(g->ctx, GALP_ENOMEM, "batch work lists");
+        }
+        // the lane-group pipeline reads entry 0 of a list even when it is
+        // empty (clamped addresses): every entry starts as vertex 0
+        for (int b = 0; b < B_HUB; ++b) {
+            const int rc = dev_fill_u32(g, (uint32_t *)B->d_alist[b], std::max<int64_t>(1, g->bucket[b].nv), 0u);
+            if (rc != GALP_OK) {
+                batch_free(g);
+                return rc;
+            }
+        }
+        int64_t members = 0;
+        for (int b = 0; b < B_HUB; ++b) members += g->bucket[b].nv;
+        g->device_bytes += 4 * members + 8 * NUM_BUCKETS;
+        B->lists = true;
     }
     return batch_reset(g);
 }

@@ -47,6 +47,13 @@ constexpr int kHubPart = 2048;        // hub partitions: ceil(deg / kHubPart)
 constexpr int kHubMaxParts = 4096;    // so hub degree <= kHubPart * kHubMaxParts
 constexpr int kHubMergeCapBits = 12;  // LDS table of one partition (4096 slots)
 
+// Parts of the neighbour-id space: (u >> kPartLineBits) mod kNbrParts, the
+// 128-byte label line of u modulo kNbrParts.  Rows stored grouped by part
+// and tiles listed part-major let the device gather from one part's lines
+// at a time (partsort.hpp, the candidate path's multi-tile rows).
+constexpr int kNbrParts = 32;
+constexpr int kPartLineBits = 5;   // 32 labels = one 128-byte line
+
 struct BucketRange {
     int64_t v0 = 0;      // first renumbered id, or first index into vlist
     int64_t nv = 0;      // vertices in the class
@@ -112,6 +119,10 @@ constexpr int kHdDiv = 8;
 constexpr int kCandTile = 1024;
 constexpr int kCandK = 16;     // candidates per tile (lanes of the candidate table, <= 64)
 constexpr int kCandCache = 38912;   // hottest labels held in LDS per workgroup (152 KB)
+// In core, the fixed tiles of multi-tile rows hold at most kCandPartTile
+// edges of a part-sorted copy of the rows (one part, or consecutive small
+// parts), listed part-major.
+constexpr int kCandPartTile = 512;
 
 // State of the learned dispatcher (allocated when dispatch is learned or
 // training samples are logged).  "Big" vertices are those of the classes
@@ -141,6 +152,8 @@ struct LearnState {
     // fixed tile list of the mt vertices: vertex, first edge, length, partial slot, mt index
     int32_t *d_mt_tv = nullptr, *d_mt_tlen = nullptr, *d_mt_tslot = nullptr, *d_mt_tmt = nullptr;
     int64_t *d_mt_te0 = nullptr;
+    int32_t *d_pnbr = nullptr;         // in core: multi-tile rows grouped by neighbour part (mt-major)
+    uint32_t *d_ppos = nullptr;        // and the row position of each of their edges
     int32_t *d_pn = nullptr;           // candidates per tile, -1 on overflow
     uint32_t *d_pkey = nullptr, *d_pcnt = nullptr, *d_plast = nullptr;
     int32_t *d_hub_sel = nullptr;      // hub left to the plain hub path this iteration
@@ -198,6 +211,14 @@ struct BatchState {
     int64_t *d_gn = nullptr, *d_vbase = nullptr;
     unsigned long long *d_gcount = nullptr;
     int32_t *d_gstop = nullptr, *d_giters = nullptr;
+    // Work lists (hash method, plain dispatch): per degree class G4..B4K,
+    // the vertices of the graphs still running, rebuilt before every
+    // iteration (k_batch_lists), with device-side lengths; the lane-group
+    // classes then run as one launch over their lists (k_group_lists).
+    bool lists = false;
+    int32_t *d_alist[NUM_BUCKETS] = {};
+    int32_t **d_alist_dev = nullptr;   // device copy of d_alist
+    int64_t *d_alen = nullptr;    // NUM_BUCKETS lengths
 };
 }  // namespace galp
 
@@ -265,6 +286,10 @@ struct galp_graph {
     galp::BatchState *batch = nullptr;
     int8_t tune[galp::NUM_BUCKETS] = {};   // kernel configuration per degree class (autotuner)
     int8_t streams = 0;   // in-core class kernels: 0 by size, 1 one stream, 3 three (the autotuner's choice)
+    // learned dispatch in core: bit i set = iteration i routes by the network,
+    // clear = by the degree classes alone (the judge's choice, autotune.hip);
+    // iterations from 63 on follow bit 63
+    uint64_t learn_mask = ~0ull;
     hipGraphExec_t gexec = nullptr;    // captured iteration loop (opts.graph_capture)
     int gexec_niter = -1;
     std::vector<int32_t> big_vid;      // host: big vertices (classes W128..HUB), hubs first
@@ -277,6 +302,28 @@ namespace galp {
 constexpr int kMaxIters = 4096;
 
 int fail(galp_ctx *ctx, int code, const std::string &msg);
+
+// hipMalloc of count elements (at least one) under the graph's memory budget;
+// counted in g->device_bytes.
+template <class T>
+int dev_alloc(galp_graph *g, T **p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    size_t bytes = sizeof(T) * (size_t)count;
+    if (g->opts.mem_budget > 0 && g->device_bytes + (int64_t)bytes > g->opts.mem_budget)
+        return fail(g->ctx, GALP_ENOMEM,
+                    "device memory budget exceeded: need " + std::to_string(g->device_bytes + (int64_t)bytes) +
+                        " bytes, budget " + std::to_string(g->opts.mem_budget));
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(g->ctx, GALP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                             hipGetErrorString(e));
+    }
+    g->device_bytes += (int64_t)bytes;
+    return GALP_OK;
+}
 
 // The whole graph as one view (in-core).
 IterView full_view(const galp_graph *g);
@@ -332,6 +379,16 @@ int ooc_features(galp_graph *g);
 // (current) once per device (mlp.hip).
 int mlp_scratch_setup(galp_ctx *ctx, int device);
 
+// Whether the iteration being launched routes by the network: learned
+// dispatch, and in core the judge's mask for this iteration (out of core
+// every batch routes by the network).
+inline bool learned_now(const galp_graph *g) {
+    if (g->opts.dispatch != GALP_DISPATCH_LEARNED || !g->learn) return false;
+    if (g->ooc) return true;
+    const int i = g->iter < 63 ? g->iter : 63;
+    return ((g->learn_mask >> i) & 1ull) != 0;
+}
+
 // Drops the captured iteration loop after a change to what it launches.
 inline void invalidate_captured(galp_graph *g) {
     if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
@@ -343,6 +400,10 @@ inline void invalidate_captured(galp_graph *g) {
 void batch_free(galp_graph *g);
 int batch_reset(galp_graph *g);
 int launch_batch_end(galp_graph *g);
+// Rebuilds the batch's work lists from the graphs' stop flags (start of an
+// iteration); a no-op without lists.
+int launch_batch_lists(galp_graph *g);
+inline bool batch_lists(const galp_graph *g) { return g->batch && g->batch->lists; }
 
 // Learned dispatch (learned.hip).
 int learn_setup(galp_graph *g);

@@ -34,6 +34,7 @@
 #include "internal.hpp"
 #include "mlp.hpp"
 #include "pipeline.hpp"
+#include "partsort.hpp"
 
 namespace galp {
 
@@ -230,8 +231,11 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
 
 // ---------------------------------------------------------------------------
 // Candidate path.  One wave per tile (pipelined over the wave's tiles);
-// candidates live in wave-uniform registers.
-template <int EPT, typename E>
+// candidates live in wave-uniform registers.  POS: the tile's edges come
+// from a part-sorted copy with their row positions alongside (the fixed
+// tiles of multi-tile rows in core), so a label's last position is the max
+// over its lanes instead of its highest lane's.
+template <int EPT, typename E, bool POS = false>
 struct CandPolicy {
     struct Item {
         bool valid;
@@ -266,7 +270,7 @@ struct CandPolicy {
             it.e0 = cw_e0[t];
             it.len = cw_len[t];
             it.pslot = cw_pslot[t];
-            it.row0 = (uint32_t)(it.e0 - (int64_t)off[it.v]);
+            if constexpr (!POS) it.row0 = (uint32_t)(it.e0 - (int64_t)off[it.v]);
             it.old = prev[it.v];
             it.valid = true;
             if (tile_mt && !mt_sel[tile_mt[t]]) it.len = 0;   // left to the class kernels
@@ -284,6 +288,12 @@ struct CandPolicy {
     // its count and last position), looked up among the candidates with one
     // more ballot.  A converged row costs one round per chunk.
     __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT]) {
+        uint32_t none[EPT];
+        process(it, lab, ids, none);
+    }
+    // rpos: the row positions of the item's elements (POS only)
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT],
+                            const uint32_t (&rpos)[EPT]) {
         if (it.len == 0) return;
         const int lane = lane_id();
         int32_t ckv = -1;          // candidate label of this lane (-1: none)
@@ -303,7 +313,9 @@ struct CandPolicy {
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(L == Lk);
                 ball &= ~m;
                 const uint32_t c = (uint32_t)__popcll(m);
-                const uint32_t lastp = rowpos + (uint32_t)(63 - __clzll((long long)m));
+                uint32_t lastp;
+                if constexpr (POS) lastp = wave_max(L == Lk ? rpos[k] : 0u);
+                else lastp = rowpos + (uint32_t)(63 - __clzll((long long)m));
                 const unsigned long long hit = __builtin_amdgcn_ballot_w64(ckv == Lk);
                 if (hit) {
                     if (lane == __ffsll((long long)hit) - 1) {
@@ -402,6 +414,58 @@ __device__ __forceinline__ void cand_pipeline(Policy &pol, const int32_t *__rest
     }
 }
 
+// cand_pipeline for POS tiles: the row positions (pos_src) are loaded with
+// each item's labels, into the same ping-pong slot.
+template <int EPT, class Policy>
+__device__ __forceinline__ void cand_pipeline_pos(Policy &pol, const int32_t *__restrict__ nbr,
+                                                  const uint32_t *__restrict__ pos_src,
+                                                  const int32_t *__restrict__ prev, int64_t first, int64_t stride,
+                                                  int32_t hot) {
+    using Item = typename Policy::Item;
+    const int lane = lane_id();
+    int64_t i = first;
+    auto load_nbr = [&](int32_t (&u)[EPT], const Item &it) {
+        const int64_t e0 = it.valid ? it.e0 : 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * kWave + lane;
+            const bool ok = it.valid && pos < it.len;
+            u[k] = __builtin_nontemporal_load(&nbr[ok ? e0 + pos : 0]);
+        }
+    };
+    auto gather = [&](int32_t (&lab)[EPT], int32_t (&ids)[EPT], uint32_t (&ps)[EPT], const int32_t (&u)[EPT],
+                      const Item &it) {
+        const int64_t e0 = it.valid ? it.e0 : 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int pos = k * kWave + lane;
+            const bool ok = it.valid && pos < it.len;
+            ids[k] = ok ? u[k] : 0;
+            lab[k] = prev[(ok && u[k] >= hot) ? u[k] : 0];
+            ps[k] = __builtin_nontemporal_load(&pos_src[ok ? e0 + pos : 0]);
+        }
+    };
+    Item s0 = pol.item(i), s1 = pol.item(i + stride), s2 = pol.item(i + 2 * stride);
+    int32_t u[EPT], labA[EPT], labB[EPT], idA[EPT], idB[EPT];
+    uint32_t pA[EPT], pB[EPT];
+    load_nbr(u, s0);
+    gather(labA, idA, pA, u, s0);
+    load_nbr(u, s1);
+    while (s0.valid) {
+        gather(labB, idB, pB, u, s1);
+        load_nbr(u, s2);
+        Item s3 = pol.item(i + 3 * stride);
+        pol.process(s0, labA, idA, pA);
+        s0 = s1; s1 = s2; s2 = s3; i += stride;
+        if (!s0.valid) break;
+        gather(labA, idA, pA, u, s1);
+        load_nbr(u, s2);
+        s3 = pol.item(i + 3 * stride);
+        pol.process(s0, labB, idB, pB);
+        s0 = s1; s1 = s2; s2 = s3; i += stride;
+    }
+}
+
 // Candidate work lists, walked in turn by every wave of one launch (one
 // LDS cache fill): the fixed tiles of multi-tile vertices, then single-tile
 // vertices by length class.
@@ -410,12 +474,18 @@ struct CandLists {
     const int64_t *e0[kCandClasses + 1];
     const int32_t *tile_mt;   // fixed list: tile -> multi-tile vertex
     int64_t nfixed;
+    // in core, the fixed tiles cover a part-sorted copy of the multi-tile
+    // rows (POS tiles of at most kCandPartTile edges); null out of core
+    const int32_t *fnbr;
+    const uint32_t *fpos;
 };
 
-// One block per CU: the LDS cache takes 152 KB of the CU's 160 KB.  At 16
-// waves the kernel is held to 128 VGPRs and spills 41 of them to scratch;
-// 512 threads (142 VGPRs, no spill, 8 waves) ran 16 % slower on RMAT-22.
-constexpr int kCandThreads = 1024;
+// One block per CU: the LDS cache takes 152 KB of the CU's 160 KB.  768
+// threads (12 waves) leave 168 VGPRs per lane: the kernel needs 142 and
+// spills nothing.  At 1024 threads it was held to 128 VGPRs and spilled 41
+// to scratch, and ran no faster (RMAT-26: 9.39 against 9.40 ms per launch);
+// 512 threads (8 waves) ran 16 % slower on RMAT-22.
+constexpr int kCandThreads = 768;
 
 template <typename E>
 __global__ void __launch_bounds__(kCandThreads)
@@ -448,7 +518,17 @@ k_cand(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t
         changed += pol.changed;
     };
     constexpr int kTileEPT = kCandTile / kWave;   // slots per lane of a full tile
-    if (nmt) run(std::integral_constant<int, kTileEPT>{}, kCandClasses, cl.nfixed, cl.tile_mt);
+    if (nmt && cl.fpos) {   // fixed POS tiles, in their part-major order
+        constexpr int EPT = kCandPartTile / kWave;
+        constexpr int li = kCandClasses;
+        CandPolicy<EPT, E, true> pol{off,    prev,  next,  cl.v[li], cl.len[li], cl.pslot[li], cl.e0[li],
+                                     cl.nfixed, pkey, pcnt, plast,   pn,         dist_out,     plist,
+                                     ctr,    cache, hot,   cl.tile_mt, mt_sel,   0u};
+        cand_pipeline_pos<EPT>(pol, cl.fnbr, cl.fpos, prev, wave, nwaves, hot);
+        changed += pol.changed;
+    } else if (nmt) {
+        run(std::integral_constant<int, kTileEPT>{}, kCandClasses, cl.nfixed, cl.tile_mt);
+    }
     if (kCandTile > 512) run(std::integral_constant<int, kTileEPT>{}, 3, nt[3], nullptr);
     run(std::integral_constant<int, 8>{}, 2, nt[2], nullptr);
     run(std::integral_constant<int, 4>{}, 1, nt[1], nullptr);
@@ -660,13 +740,79 @@ void learn_free(galp_graph *g) {
                     L->d_plast,   L->d_hub_sel,    L->d_hub_distinct, L->d_plist, L->d_plist_n, L->d_log_x,
                     L->d_log_y,   L->d_log_n, L->d_cand_bytes, L->d_cw_base,
                     L->d_mt_tv,   L->d_mt_te0,     L->d_mt_tlen,  L->d_mt_tslot, L->d_mt_tmt,
-                    L->d_hub_hd};
+                    L->d_hub_hd,  L->d_pnbr,       L->d_ppos};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int b = 0; b < 2 * NUM_BUCKETS; ++b)
         if (L->plist[b]) (void)hipFree(L->plist[b]);
     delete L;
     g->learn = nullptr;
+}
+
+// In core: the part-sorted copy of the multi-tile rows (L->d_pnbr,
+// L->d_ppos; mt-major, compact) and its POS tiles of at most kCandPartTile
+// edges (partsort.hpp): slots mt-major, the list sorted by first part.
+static int fixed_part_tiles(galp_graph *g, LearnState *L, const std::vector<int32_t> &mt_vid,
+                            const std::vector<int64_t> &mt_deg, std::vector<int32_t> &ctile0,
+                            std::vector<int32_t> &nctiles, std::vector<int32_t> &tv, std::vector<int64_t> &te0,
+                            std::vector<int32_t> &tlen, std::vector<int32_t> &tslot, std::vector<int32_t> &tmt) {
+    const int64_t nr = (int64_t)mt_vid.size();
+    std::vector<int64_t> roff((size_t)nr + 1, 0);
+    for (int64_t r = 0; r < nr; ++r) roff[(size_t)r + 1] = roff[(size_t)r] + mt_deg[(size_t)r];
+    GALP_LTRY(lalloc(g, &L->d_pnbr, roff.back()));
+    GALP_LTRY(lalloc(g, &L->d_ppos, roff.back()));
+    int32_t *d_rows = nullptr, *d_count = nullptr;
+    int64_t *d_roff = nullptr;
+    std::vector<int32_t> count((size_t)(nr * kNbrParts));
+    hipError_t e = hipMalloc((void **)&d_rows, 4 * (size_t)nr);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_roff, 8 * (size_t)(nr + 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_count, 4 * count.size());
+    if (e == hipSuccess) e = hipMemcpy(d_rows, mt_vid.data(), 4 * (size_t)nr, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_roff, roff.data(), 8 * roff.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const int grid = (int)((nr + 3) / 4);
+        if (g->off64)
+            k_rows_partsort<int64_t><<<grid, 256, 0, g->stream>>>((const int64_t *)g->d_off, g->d_nbr, d_rows, nr,
+                                                                  d_roff, L->d_pnbr, L->d_ppos, d_count);
+        else
+            k_rows_partsort<int32_t><<<grid, 256, 0, g->stream>>>((const int32_t *)g->d_off, g->d_nbr, d_rows, nr,
+                                                                  d_roff, L->d_pnbr, L->d_ppos, d_count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e == hipSuccess) e = hipMemcpy(count.data(), d_count, 4 * count.size(), hipMemcpyDeviceToHost);
+    if (d_rows) (void)hipFree(d_rows);
+    if (d_roff) (void)hipFree(d_roff);
+    if (d_count) (void)hipFree(d_count);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(g->ctx, GALP_EHIP, std::string("multi-tile part sort: ") + hipGetErrorString(e));
+    }
+    std::vector<int32_t> part;
+    ctile0.assign((size_t)nr, 0);
+    nctiles.assign((size_t)nr, 0);
+    part_tiles(nr, roff, count, kCandPartTile, [&](int64_t r, int64_t e0, int64_t len, int p) {
+        if (nctiles[(size_t)r]++ == 0) ctile0[(size_t)r] = (int32_t)tv.size();
+        tslot.push_back((int32_t)tv.size());
+        tv.push_back(mt_vid[(size_t)r]);
+        te0.push_back(e0);
+        tlen.push_back((int32_t)len);
+        tmt.push_back((int32_t)r);
+        part.push_back(p);
+    });
+    std::vector<size_t> ord(tv.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return part[a] < part[b]; });
+    auto permute = [&](auto &v) {
+        auto c = v;
+        for (size_t i = 0; i < ord.size(); ++i) v[i] = c[ord[i]];
+    };
+    permute(tv);
+    permute(te0);
+    permute(tlen);
+    permute(tslot);
+    permute(tmt);
+    return GALP_OK;
 }
 
 // Allocates the learned-dispatch state and computes the static features.
@@ -701,16 +847,42 @@ int learn_setup(galp_graph *g) {
         return fail(g->ctx, GALP_EHIP, "class list table upload failed");
     // multi-tile vertices: hubs first (g->big_vid lists hubs first)
     std::vector<int32_t> mt_vid, ctile0, nctiles, mt_of((size_t)n, -1);
-    int64_t nct = 0;
+    std::vector<int64_t> mt_deg, mt_off;
     for (size_t k = 0; k < g->big_vid.size(); ++k) {
         const int64_t d = g->big_deg[k];
         if (d <= kCandTile) continue;
         mt_of[(size_t)g->big_vid[k]] = (int32_t)mt_vid.size();
         mt_vid.push_back(g->big_vid[k]);
-        const int32_t nt = (int32_t)((d + kCandTile - 1) / kCandTile);
-        ctile0.push_back((int32_t)nct);
-        nctiles.push_back(nt);
-        nct += nt;
+        mt_deg.push_back(d);
+        mt_off.push_back(g->big_off[k]);
+    }
+    // the fixed tile list: vertex, first edge, length, partial slot,
+    // multi-tile index per tile; slots are consecutive per vertex (the
+    // merge's ranges), the list itself is in processing order
+    std::vector<int32_t> tv, tlen, tslot, tmt;
+    std::vector<int64_t> te0;
+    int64_t nct = 0;
+    if (!g->ooc && !mt_vid.empty()) {
+        // in core: POS tiles over a part-sorted copy of the rows, listed
+        // part-major so the device gathers from one part's lines at a time
+        GALP_LTRY(fixed_part_tiles(g, L, mt_vid, mt_deg, ctile0, nctiles, tv, te0, tlen, tslot, tmt));
+        nct = (int64_t)tv.size();
+    } else {
+        // out of core (only row-order tiles of kCandTile edges over the batch
+        // buffers; the dispatch leaves multi-tile rows to the class paths)
+        for (size_t r = 0; r < mt_vid.size(); ++r) {
+            ctile0.push_back((int32_t)nct);
+            int32_t k = 0;
+            for (int64_t t = 0; t * kCandTile < mt_deg[r]; ++t, ++k) {
+                tslot.push_back((int32_t)tv.size());
+                tv.push_back(mt_vid[r]);
+                te0.push_back(mt_off[r] + t * kCandTile);
+                tlen.push_back((int32_t)std::min<int64_t>(kCandTile, mt_deg[r] - t * kCandTile));
+                tmt.push_back((int32_t)r);
+            }
+            nctiles.push_back(k);
+            nct += k;
+        }
     }
     for (int64_t h = 0; h < g->nhubs; ++h)   // hub h is multi-tile vertex h
         if (mt_of[(size_t)g->big_vid[(size_t)h]] != (int32_t)h)
@@ -737,21 +909,7 @@ int learn_setup(galp_graph *g) {
     GALP_LTRY(lalloc(g, &L->d_mt_ctile0, L->nmt));
     GALP_LTRY(lalloc(g, &L->d_mt_nctiles, L->nmt));
     GALP_LTRY(lalloc(g, &L->d_mt_cand, L->nmt));
-    {   // the fixed tile list of multi-tile vertices
-        std::vector<int32_t> tv, tlen, tslot, tmt;
-        std::vector<int64_t> te0;
-        for (size_t k = 0; k < g->big_vid.size(); ++k) {
-            const int64_t d = g->big_deg[k];
-            if (d <= kCandTile) continue;
-            const int32_t mt = mt_of[(size_t)g->big_vid[k]];
-            for (int64_t t = 0; t * kCandTile < d; ++t) {
-                tslot.push_back((int32_t)tv.size());
-                tv.push_back(g->big_vid[k]);
-                te0.push_back(g->big_off[k] + t * kCandTile);
-                tlen.push_back((int32_t)std::min<int64_t>(kCandTile, d - t * kCandTile));
-                tmt.push_back(mt);
-            }
-        }
+    {   // the fixed tile list
         GALP_LTRY(lalloc(g, &L->d_mt_tv, nct));
         GALP_LTRY(lalloc(g, &L->d_mt_te0, nct));
         GALP_LTRY(lalloc(g, &L->d_mt_tlen, nct));
@@ -892,6 +1050,8 @@ static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_
     cl.e0[kCandClasses] = L->d_mt_te0;
     cl.tile_mt = L->d_mt_tmt;
     cl.nfixed = L->nctiles;
+    cl.fnbr = g->ooc ? nullptr : L->d_pnbr;
+    cl.fpos = g->ooc ? nullptr : L->d_ppos;
     k_cand<E><<<g->ctx->prop.multiProcessorCount, kCandThreads, 0, g->stream>>>(
         off, nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, L->distinct_out(g), dpl,
         L->d_plist_n, hot, g->d_counter, g->d_stop);

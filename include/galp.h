@@ -247,7 +247,7 @@ GALP_API int galp_graph_log_read(galp_graph *g, float *x, float *y, int64_t max,
 /* Raw static features per vertex (original numbering): degree, mean and max
  * neighbour degree, computed for vertices of degree > 64 (0 otherwise). */
 GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *mx);
-/* Counters of the last iteration's learned dispatch: [0..12) class list
+/* Counters of the last iteration routed by the network: [0..12) class list
  * sizes, [12..16) candidate tiles per length class (<= 128, 256, 512, 1024
  * edges), [16] candidate overflows, [17] vertices routed to candidates,
  * [18] hubs left to the plain hub path. */
@@ -262,15 +262,26 @@ GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max,
  * rounds, reads each class's device time from per-kernel events and keeps
  * its fastest option; then, in core, it times such runs with the class
  * kernels on one stream and on three and keeps the size rule's choice
- * unless the other is more than 3 % faster.  It leaves the graph
+ * unless the other is more than 3 % faster; under learned dispatch it then
+ * judges the network against the plain classes per iteration
+ * (galp_graph_set_learned_iterations).  It leaves the graph
  * initialised (labels[i] = i) and the per-kernel profile totals as they
  * were.  cache_path (may be NULL) names a text file of earlier
  * choices keyed by device arch, method and the class's log2 sizes; hits are
  * reused without timing and new choices are appended.  chosen (may be NULL)
  * receives 16 entries: the 12 class options, then the stream choice (1 or 3;
- * 0 where not timed), then zeros.  Labels never depend on the choices.  No
+ * 0 where not timed), then the judge's mask (two halves; zeros without
+ * learned dispatch), then zeros.  Labels never depend on the choices.  No
  * reference counterpart (fixed block sizes, lfht.cuh:127-165). */
 GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_path, int32_t *chosen);
+/* Learned dispatch in core: which iterations route by the network (bit i
+ * for iteration i, bit 63 for every later one); the others run the plain
+ * degree classes, the dispatcher the network is judged against.
+ * galp_graph_autotune sets it by timing both per iteration (chosen[13] and
+ * chosen[14] receive its low and high halves); the default routes every
+ * iteration by the network.  Labels never depend on it.  No reference
+ * counterpart. */
+GALP_API int galp_graph_set_learned_iterations(galp_graph *g, uint64_t mask);
 /* Sets the option of every class directly (12 entries, each 0..2). */
 GALP_API int galp_graph_set_tuning(galp_graph *g, const int32_t *options);
 

@@ -10,7 +10,9 @@
   any count, so multi-label rows overflow and take the spill path), nothing,
   and seeded random networks in between, in core and out of core;
 - what a network trained on the GPU gains over plain dispatch on RMAT-22
-  (same labels, less device time).
+  (same labels, less device time);
+- the judge's per-iteration choice between the network and the plain
+  classes: labels exact under any mask, and under the autotuner's.
 """
 import sys
 
@@ -138,13 +140,15 @@ def test_logged_distinct_counts_exact(ctx, reorder):
             prev, _ = oracle.lp_iterate(off, nbr, prev)
 
 
-def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, reorder=1, **opts):
+def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, reorder=1, mask=None, **opts):
     say(f"[learned] {tag} n={len(off) - 1} m={len(nbr)} thresh={thresh} reorder={reorder} {opts}")
     n = len(off) - 1
     prev = np.arange(n, dtype=np.int32) if prev is None else prev
     routed = overflowed = 0
     with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, reorder=reorder, **opts) as g:
         g.set_mlp(hidden, params, thresh)
+        if mask is not None:
+            g.set_learned_iterations(mask)
         g.set_labels(prev)
         for it in range(niter):
             ref, cnt = oracle.lp_iterate(off, nbr, prev)
@@ -156,10 +160,13 @@ def run_learned(ctx, off, nbr, niter, tag, params, hidden, thresh, prev=None, re
                                      f"{bad[0]} (deg {off[bad[0] + 1] - off[bad[0]]}): gpu {got[bad[0]]} "
                                      f"oracle {ref[bad[0]]}")
             assert st["updated"] == cnt, (tag, it + 1, st["updated"], cnt)
-            ds = g.dispatch_stats()
-            routed += int(ds[_abi.DS_CAND_VERTS])
-            overflowed += int(ds[_abi.DS_OVERFLOW])
-            assert st["overflowed"] == ds[_abi.DS_OVERFLOW]
+            if mask is None or (mask >> min(it, 63)) & 1:   # routed by the network
+                ds = g.dispatch_stats()
+                routed += int(ds[_abi.DS_CAND_VERTS])
+                overflowed += int(ds[_abi.DS_OVERFLOW])
+                assert st["overflowed"] == ds[_abi.DS_OVERFLOW]
+            else:   # the plain classes: no dispatch, no overflow
+                assert st["overflowed"] == 0
             prev = ref
             if st["stopped"]:
                 break
@@ -205,6 +212,39 @@ def test_learned_random_networks(ctx, seed):
     off, nbr = _abi.gen_rmat(16, 16, seed=seed)
     p = _abi.mlp_init(32, seed)
     run_learned(ctx, off, nbr, 8, f"rmat16-rand{seed}", p, 32, 3.0)
+
+
+@pytest.mark.parametrize("mask", [0x5555555555555555, 0xAAAAAAAAAAAAAAAA, 0])
+def test_learned_iteration_mask(ctx, mask):
+    # iterations routed by the network and by the plain classes, alternating
+    # (or never by the network): the labels stay the oracle's every iteration
+    off, nbr = big_graph(9)
+    run_learned(ctx, off, nbr, 7, f"mask{mask:x}", const_net(8, 1.0), 8, 4.0, mask=mask)
+    off, nbr = _abi.gen_rmat(16, 16, seed=9)
+    run_learned(ctx, off, nbr, 7, f"rmat16-mask{mask:x}", _abi.mlp_init(32, 9), 32, 6.0, mask=mask)
+
+
+def test_judge_picks_routers_per_iteration(ctx):
+    # the autotuner's judge: a mask over the iterations, and the labels of a
+    # run under it (eager and captured) equal the oracle's
+    off, nbr = _abi.gen_rmat(17, 16, seed=17)
+    final, per, counts = oracle.lp_run(off, nbr, niter=10)
+    x, y = None, None
+    with _abi.Graph(ctx, off, nbr) as g:
+        g.log_enable(11 * int((np.diff(off) > 64).sum()))
+        g.run(10, want_labels=False)
+        x, y = g.log_read()
+    p, _ = _abi.mlp_train(ctx, 32, _abi.mlp_init(32, 1), x, y, epochs=2, batch=512, lr=3e-3, seed=2)
+    for capture in (0, 1):
+        with _abi.Graph(ctx, off, nbr, dispatch=LEARNED, graph_capture=capture) as g:
+            g.set_mlp(32, p, 14.0)
+            g.autotune(reps=1)
+            mask = g.learned_iterations
+            say(f"[learned] judge mask {mask & 0x3FF:010b} (capture {capture})")
+            for _ in range(2):
+                labels, res = g.run(10)
+                assert res["iterations"] == len(per)
+                assert np.array_equal(labels, final)
 
 
 def test_learned_device_loop(ctx):
