@@ -12,12 +12,11 @@
 // (lfht.cuh:97, dpp.cuh:88) applied per graph: stop after an iteration with
 // updated_k <= n_k * 1e-5.
 // Under the hash method with plain dispatch, the graphs that stopped also
-// leave the work: before every iteration k_batch_lists rebuilds, per degree
-// class below the hubs, the list of the running graphs' vertices (lengths
-// on the device), the LDS classes walk their lists, and the five lane-group
-// classes run as one launch over theirs (k_group_lists, lp_hash.hip), so a
-// frozen graph costs no gathers; the epilogue keeps its labels in both
-// buffers.
+// leave the work: k_batch_lists builds, per degree class below the hubs,
+// the list of the running graphs' vertices (lengths on the device) before
+// the first iteration and again after each iteration in which a graph
+// stopped, and every class kernel walks its list, so a frozen graph costs no
+// gathers; the epilogue keeps its labels in both buffers.
 
 #include <algorithm>
 #include <vector>
@@ -57,41 +56,69 @@ __global__ void k_batch_check(const E *__restrict__ off, const int32_t *__restri
 
 // The work lists: the members of classes G4..B4K (flat index over the
 // classes: class b holds [f0[b], f0[b+1])) whose graph is still running, each
-// appended to its class's list; appends are aggregated per wave and class.
+// appended to its class's list.  Appends are aggregated per block: a block
+// takes kListChunk consecutive members at a time (kListItems per thread),
+// ranks them per class in LDS, reserves each class's range with one device
+// atomic and writes.  (One atomic per wave put ~6 million same-address
+// atomics per iteration on a 400-million-vertex batch, serialised at the L2.)
 struct ClassSpans {
     int64_t v0[NUM_BUCKETS];   // first id, or first index into vlist
     int64_t f0[NUM_BUCKETS + 1];
 };
 
+constexpr int kListItems = 16;
+constexpr int64_t kListChunk = 256 * kListItems;
+
 __global__ void __launch_bounds__(256)
 k_batch_lists(ClassSpans cs, const int32_t *__restrict__ vlist, const int32_t *__restrict__ gid,
               const int32_t *__restrict__ gstop, int32_t *const *__restrict__ alist, int64_t *__restrict__ alen,
               const int32_t *__restrict__ stop) {
-    if (*stop) return;
+    __shared__ uint32_t scnt[NUM_BUCKETS];
+    __shared__ int64_t sbase[NUM_BUCKETS];
+    if (*stop || !alen[NUM_BUCKETS]) return;   // no graph stopped since the last rebuild
     const int64_t total = cs.f0[B_HUB];
-    for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
-        const int64_t i = base + threadIdx.x;
-        int b = 0;
-        bool keep = false;
-        int32_t v = 0;
-        if (i < total) {
-            while (i >= cs.f0[b + 1]) ++b;
-            const int64_t k = cs.v0[b] + (i - cs.f0[b]);
-            v = vlist ? vlist[k] : (int32_t)k;
-            keep = gstop[gid[v]] == 0;
+    const unsigned long long lt = (1ull << lane_id()) - 1ull;
+    for (int64_t c0 = (int64_t)blockIdx.x * kListChunk; c0 < total; c0 += (int64_t)gridDim.x * kListChunk) {
+        if (threadIdx.x < NUM_BUCKETS) scnt[threadIdx.x] = 0u;
+        __syncthreads();
+        int32_t v[kListItems];
+        int cls[kListItems];       // -1: not kept
+        uint32_t rank[kListItems];   // within the block's share of the class
+#pragma unroll
+        for (int r = 0; r < kListItems; ++r) {
+            const int64_t i = c0 + (int64_t)r * 256 + threadIdx.x;
+            int b = 0;
+            bool keep = false;
+            v[r] = 0;
+            if (i < total) {
+                while (i >= cs.f0[b + 1]) ++b;
+                const int64_t k = cs.v0[b] + (i - cs.f0[b]);
+                v[r] = vlist ? vlist[k] : (int32_t)k;
+                keep = gstop[gid[v[r]]] == 0;
+            }
+            cls[r] = keep ? b : -1;
+            rank[r] = 0u;
+            unsigned long long ball;
+            while ((ball = __ballot(keep)) != 0ull) {
+                const int leader = __ffsll((long long)ball) - 1;
+                const int bl = __builtin_amdgcn_readlane(b, leader);
+                const bool mine = keep && b == bl;
+                const unsigned long long m = __ballot(mine);
+                uint32_t at = 0u;
+                if (lane_id() == leader) at = atomicAdd(&scnt[bl], (uint32_t)__popcll(m));
+                at = (uint32_t)__shfl((int)at, leader);
+                if (mine) rank[r] = at + (uint32_t)__popcll(m & lt);
+                keep = keep && !mine;
+            }
         }
-        unsigned long long ball;
-        while ((ball = __ballot(keep)) != 0ull) {
-            const int leader = __ffsll((long long)ball) - 1;
-            const int bl = __builtin_amdgcn_readlane(b, leader);
-            const bool mine = keep && b == bl;
-            const unsigned long long m = __ballot(mine);
-            unsigned long long at = 0;
-            if (lane_id() == leader) at = atomicAdd((unsigned long long *)(alen + bl), (unsigned long long)__popcll(m));
-            at = __shfl(at, leader);
-            if (mine) alist[bl][at + __popcll(m & ((1ull << lane_id()) - 1ull))] = v;
-            keep = keep && !mine;
-        }
+        __syncthreads();
+        if (threadIdx.x < NUM_BUCKETS && scnt[threadIdx.x])
+            sbase[threadIdx.x] = (int64_t)atomicAdd((unsigned long long *)(alen + threadIdx.x),
+                                                    (unsigned long long)scnt[threadIdx.x]);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kListItems; ++r)
+            if (cls[r] >= 0) alist[cls[r]][sbase[cls[r]] + rank[r]] = v[r];
     }
 }
 
@@ -124,44 +151,63 @@ k_batch_epilogue(const int32_t *__restrict__ prev, int32_t *__restrict__ next, c
     }
 }
 
-// Per-graph stop rule and the run's bookkeeping (one block).
+// Before a rebuild of the work lists: zero their lengths.  alen holds the
+// NUM_BUCKETS lengths, then the rebuild flag (set when a graph stopped in
+// the last iteration; the lists stand otherwise).
+__global__ void k_batch_lists_begin(int64_t *__restrict__ alen, const int32_t *__restrict__ stop) {
+    if (*stop || !alen[NUM_BUCKETS]) return;
+    if (threadIdx.x < NUM_BUCKETS) alen[threadIdx.x] = 0;
+}
+
+// Per-graph stop rule and the run's bookkeeping (one block).  rebuild (may
+// be null) is set to whether a graph stopped in this iteration.
 __global__ void __launch_bounds__(1024)
 k_batch_stop(unsigned long long *__restrict__ gcount, const int64_t *__restrict__ gn, int32_t *__restrict__ gstop,
              int32_t *__restrict__ giters, int64_t ngraphs, unsigned long long *__restrict__ counter,
              int32_t *__restrict__ stop, long long *__restrict__ iter_counts, int32_t *__restrict__ iters_done,
-             int it) {
+             int it, int64_t *__restrict__ rebuild) {
     __shared__ unsigned long long stotal;
-    __shared__ int32_t sactive;
+    __shared__ int32_t sactive, sstopped;
     if (*stop) return;
     if (threadIdx.x == 0) {
         stotal = 0;
         sactive = 0;
+        sstopped = 0;
     }
     __syncthreads();
     unsigned long long tot = 0;
-    int active = 0;
+    int active = 0, stopped = 0;
     for (int64_t k = threadIdx.x; k < ngraphs; k += 1024) {
         const unsigned long long c = gcount[k];
         gcount[k] = 0ull;
         if (gstop[k]) continue;
         tot += c;
         giters[k] = it + 1;
-        if ((double)c <= (double)gn[k] * 1e-5) gstop[k] = 1;
-        else active = 1;
+        if ((double)c <= (double)gn[k] * 1e-5) {
+            gstop[k] = 1;
+            stopped = 1;
+        } else {
+            active = 1;
+        }
     }
     atomicAdd(&stotal, tot);
     if (active) sactive = 1;
+    if (stopped) sstopped = 1;
     __syncthreads();
     if (threadIdx.x == 0) {
         iter_counts[it] = (long long)stotal;
         *iters_done = it + 1;
         *counter = 0ull;   // the class kernels' whole-graph count is not used in batched runs
         if (!sactive) *stop = 1;
+        if (rebuild) *rebuild = sstopped;
     }
 }
 
+// rebuild (may be null): the work lists' rebuild flag, set for the first
+// iteration.
 __global__ void k_batch_reset(int32_t *__restrict__ gstop, int32_t *__restrict__ giters,
-                              unsigned long long *__restrict__ gcount, int64_t ngraphs) {
+                              unsigned long long *__restrict__ gcount, int64_t ngraphs, int64_t *__restrict__ rebuild) {
+    if (rebuild && blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 1;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ngraphs; k += (int64_t)gridDim.x * blockDim.x) {
         gstop[k] = 0;
         giters[k] = 0;
@@ -185,7 +231,7 @@ int batch_reset(galp_graph *g) {
     BatchState *B = g->batch;
     if (!B) return GALP_OK;
     k_batch_reset<<<(int)std::min<int64_t>((B->ngraphs + 255) / 256, 1024), 256, 0, g->stream>>>(
-        B->d_gstop, B->d_giters, B->d_gcount, B->ngraphs);
+        B->d_gstop, B->d_giters, B->d_gcount, B->ngraphs, B->lists ? B->d_alen + NUM_BUCKETS : nullptr);
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "batch reset failed");
 }
 
@@ -198,10 +244,9 @@ int launch_batch_lists(galp_graph *g) {
         cs.v0[b] = g->bucket[b].v0;
         cs.f0[b + 1] = cs.f0[b] + (b < B_HUB ? g->bucket[b].nv : 0);
     }
-    int rc = dev_fill_u32(g, (uint32_t *)B->d_alen, 2 * NUM_BUCKETS, 0u);
-    if (rc != GALP_OK) return rc;
+    k_batch_lists_begin<<<1, 64, 0, g->stream>>>(B->d_alen, g->d_stop);
     const int64_t total = cs.f0[B_HUB];
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256,
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((total + kListChunk - 1) / kListChunk,
                                                                  (int64_t)g->ctx->prop.multiProcessorCount * 8));
     k_batch_lists<<<grid, 256, 0, g->stream>>>(cs, g->d_vlist, B->d_gid, B->d_gstop, B->d_alist_dev, B->d_alen,
                                                g->d_stop);
@@ -217,7 +262,8 @@ int launch_batch_end(galp_graph *g) {
                                                                  (int64_t)g->ctx->prop.multiProcessorCount * 8));
     k_batch_epilogue<<<grid, 256, 0, g->stream>>>(prev, next, B->d_gid, B->d_gstop, B->d_gcount, g->n, g->d_stop);
     k_batch_stop<<<1, 1024, 0, g->stream>>>(B->d_gcount, B->d_gn, B->d_gstop, B->d_giters, B->ngraphs, g->d_counter,
-                                            g->d_stop, g->d_iter_counts, g->d_iters_done, g->iter % kMaxIters);
+                                            g->d_stop, g->d_iter_counts, g->d_iters_done, g->iter % kMaxIters,
+                                            B->lists ? B->d_alen + NUM_BUCKETS : nullptr);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, std::string("batch epilogue: ") + hipGetErrorString(e));
 }
@@ -288,7 +334,7 @@ GALP_API int galp_graph_set_batch(galp_graph *g, int64_t ngraphs, const int64_t 
     }
     if (g->opts.method == GALP_METHOD_HASH && g->opts.dispatch == GALP_DISPATCH_PLAIN) {
         // work lists of the running graphs' vertices, one per class below the hubs
-        bool okl = hipMalloc((void **)&B->d_alen, 8 * NUM_BUCKETS) == hipSuccess &&
+        bool okl = hipMalloc((void **)&B->d_alen, 8 * (NUM_BUCKETS + 1)) == hipSuccess &&
                    hipMalloc((void **)&B->d_alist_dev, sizeof(int32_t *) * NUM_BUCKETS) == hipSuccess;
         for (int b = 0; b < B_HUB && okl; ++b)
             okl = hipMalloc((void **)&B->d_alist[b], 4 * (size_t)std::max<int64_t>(1, g->bucket[b].nv)) == hipSuccess;
@@ -311,7 +357,7 @@ GALP_API int galp_graph_set_batch(galp_graph *g, int64_t ngraphs, const int64_t 
         }
         int64_t members = 0;
         for (int b = 0; b < B_HUB; ++b) members += g->bucket[b].nv;
-        g->device_bytes += 4 * members + 8 * NUM_BUCKETS;
+        g->device_bytes += 4 * members + 8 * (NUM_BUCKETS + 1);
         B->lists = true;
     }
     return batch_reset(g);

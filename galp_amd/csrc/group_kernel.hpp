@@ -85,15 +85,18 @@ __device__ __forceinline__ uint32_t group_rounds(const E *__restrict__ off, cons
 // Alt: the resolver of iterations where most rows are high-distinct (the
 // learned dispatcher's count of such big vertices, *hd_count, exceeds half of
 // hd_total); chosen once per launch, so each choice is its own pipeline.
+// nv_dev, when given, holds the list's length (written on the device by an
+// earlier kernel; nv is then only its capacity).
 template <int G, class Resolve, class Alt, bool VLIST, typename E>
 __global__ void __launch_bounds__(256)
 k_group(const E *__restrict__ off, const int32_t *__restrict__ nbr,
         const int32_t *__restrict__ prev, int32_t *__restrict__ next,
-        const int32_t *__restrict__ vlist, int64_t v0, int64_t nv,
+        const int32_t *__restrict__ vlist, int64_t v0, int64_t nv, const int64_t *__restrict__ nv_dev,
         unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
         const int64_t *__restrict__ hd_count, int64_t hd_total) {
     __shared__ uint32_t scratch[256 / kWave];
     if (*stop) return;
+    if (nv_dev) nv = *nv_dev;
     const bool alt = hd_count && 2 * (*hd_count) > hd_total;   // uniform
     const uint32_t changed = alt ? group_rounds<G, Alt, VLIST, E>(off, nbr, prev, next, vlist, v0, nv)
                                  : group_rounds<G, Resolve, VLIST, E>(off, nbr, prev, next, vlist, v0, nv);

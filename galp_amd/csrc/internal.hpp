@@ -171,7 +171,6 @@ struct LearnState {
     int64_t log_cap = 0;
     float *d_log_x = nullptr, *d_log_y = nullptr;
     int64_t *d_log_n = nullptr;
-    int32_t *distinct_out(const galp_graph *) const { return d_dist[dcur ^ 1]; }
 };
 // State of the sort method: the 8-bit radix passes its LDS sorts make for
 // labels < n.  Its hub path uses the hub plan both methods share.
@@ -212,13 +211,14 @@ struct BatchState {
     unsigned long long *d_gcount = nullptr;
     int32_t *d_gstop = nullptr, *d_giters = nullptr;
     // Work lists (hash method, plain dispatch): per degree class G4..B4K,
-    // the vertices of the graphs still running, rebuilt before every
-    // iteration (k_batch_lists), with device-side lengths; the lane-group
-    // classes then run as one launch over their lists (k_group_lists).
+    // the vertices of the graphs still running, rebuilt before the first
+    // iteration and after every iteration in which a graph stopped
+    // (k_batch_lists), with device-side lengths, which the class kernels walk
+    // instead of the class vertex ranges.
     bool lists = false;
     int32_t *d_alist[NUM_BUCKETS] = {};
     int32_t **d_alist_dev = nullptr;   // device copy of d_alist
-    int64_t *d_alen = nullptr;    // NUM_BUCKETS lengths
+    int64_t *d_alen = nullptr;    // NUM_BUCKETS lengths, then the rebuild flag
 };
 }  // namespace galp
 
@@ -387,6 +387,19 @@ inline bool learned_now(const galp_graph *g) {
     if (g->ooc) return true;
     const int i = g->iter < 63 ? g->iter : 63;
     return ((g->learn_mask >> i) & 1ull) != 0;
+}
+
+// Where this iteration's paths record each big row's distinct-label count:
+// the next iteration's dispatch input and the logged training target, so
+// nowhere (null) when the next iteration routes by the degree classes alone
+// and nothing is logged.
+inline int32_t *distinct_out(const galp_graph *g) {
+    const LearnState *L = g->learn;
+    if (!L) return nullptr;
+    const int i = g->iter + 1 < 63 ? g->iter + 1 : 63;
+    const bool next_learned = g->opts.dispatch == GALP_DISPATCH_LEARNED &&
+                              (g->ooc || ((g->learn_mask >> i) & 1ull) != 0);
+    return next_learned || L->log_cap ? L->d_dist[L->dcur ^ 1] : nullptr;
 }
 
 // Drops the captured iteration loop after a change to what it launches.

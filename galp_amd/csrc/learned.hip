@@ -1053,14 +1053,14 @@ static int learned_front(galp_graph *g, const E *off, const int32_t *nbr, int64_
     cl.fnbr = g->ooc ? nullptr : L->d_pnbr;
     cl.fpos = g->ooc ? nullptr : L->d_ppos;
     k_cand<E><<<g->ctx->prop.multiProcessorCount, kCandThreads, 0, g->stream>>>(
-        off, nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, L->distinct_out(g), dpl,
+        off, nbr, prev, next, cl, L->d_mt_cand, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, distinct_out(g), dpl,
         L->d_plist_n, hot, g->d_counter, g->d_stop);
     prof_end(g, &pl);
     if (merge && L->nmt && !g->ooc) {
         prof_begin(g, "cand_merge", 0.0, &pl);
         k_cand_merge<E><<<grid_cap(g, (L->nmt + 3) / 4, 8), 256, 0, g->stream>>>(
             (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
-            L->d_mt_nctiles, dpl, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, prev, next, L->distinct_out(g),
+            L->d_mt_nctiles, dpl, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, prev, next, distinct_out(g),
             g->d_counter, L->d_plist_n, g->d_stop);
         prof_end(g, &pl);
     }
@@ -1083,7 +1083,7 @@ static int learned_merge(galp_graph *g) {
     k_cand_merge<E><<<grid_cap(g, (L->nmt + 3) / 4, 8), 256, 0, g->stream>>>(
         (const E *)g->d_off, L->d_mt_vid, L->nmt, g->nhubs, L->d_mt_cand, L->d_hub_sel, L->d_mt_ctile0,
         L->d_mt_nctiles, L->d_plist, L->d_pkey, L->d_pcnt, L->d_plast, L->d_pn, g->d_lab[g->cur],
-        g->d_lab[g->cur ^ 1], L->distinct_out(g), g->d_counter, L->d_plist_n, g->d_stop);
+        g->d_lab[g->cur ^ 1], distinct_out(g), g->d_counter, L->d_plist_n, g->d_stop);
     prof_end(g, &pl);
     return hipGetLastError() == hipSuccess ? GALP_OK : fail(g->ctx, GALP_EHIP, "candidate merge launch failed");
 }

@@ -671,13 +671,18 @@ static void launch_group(galp_graph *g, const IterView &w, int bid, const int32_
     const bool learned = learned_now(g);
     const int64_t *hd = learned ? g->learn->d_plist_n + kCtrHdAll : nullptr;
     const int64_t hd_total = learned ? g->learn->front_n : 0;   // big vertices of this dispatch
-    if (g->d_vlist)
+    if (batch_lists(g))   // the running graphs' vertices of the class
         k_group<G, HashGroupResolve, HashGroupSweep, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, hd, hd_total);
+            (const E *)w.off, w.nbr, prev, next, g->batch->d_alist[bid], 0, r.nv, g->batch->d_alen + bid,
+            g->d_counter, g->d_stop, hd, hd_total);
+    else if (g->d_vlist)
+        k_group<G, HashGroupResolve, HashGroupSweep, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0, g->stream>>>(
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop, hd,
+            hd_total);
     else
         k_group<G, HashGroupResolve, HashGroupSweep, false, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0,
                                                                  g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop, hd, hd_total);
+            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, nullptr, g->d_counter, g->d_stop, hd, hd_total);
     prof_end(g, &pl);
 }
 
@@ -688,7 +693,7 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
     if (!r.nv) return;
     PendingLaunch pl;
     prof_begin(g, kBucketName[bid], class_bytes<E>(r), &pl);
-    int32_t *dist = g->learn ? g->learn->distinct_out(g) : nullptr;
+    int32_t *dist = distinct_out(g);
     if (batch_lists(g))   // the running graphs' vertices of the class
         k_hash_lds<T, EPT, CAPBITS, E, 3><<<grid_for(g, r.nv, blocks_per_cu), T, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, next, g->batch->d_alist[bid], 0, r.nv, g->batch->d_alen + bid, nullptr,
@@ -705,49 +710,6 @@ static void launch_lds(galp_graph *g, const IterView &w, int bid, const int32_t 
     prof_end(g, &pl);
 }
 
-// Batched runs with work lists: the five lane-group classes in one launch,
-// every wave walking the G4, G8, .., G64 lists in turn (the lists hold only
-// the vertices of graphs still running; lengths on the device).
-struct GroupLists {
-    const int32_t *list[B_G64 + 1];
-    const int64_t *len;
-};
-
-template <typename E>
-__global__ void __launch_bounds__(256)
-k_group_lists(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
-              int32_t *__restrict__ next, GroupLists gl, unsigned long long *__restrict__ counter,
-              const int32_t *__restrict__ stop) {
-    __shared__ uint32_t scratch[256 / kWave];
-    if (*stop) return;
-    uint32_t changed = 0;
-    changed += group_rounds<64, HashGroupResolve, true, E>(off, nbr, prev, next, gl.list[B_G64], 0, gl.len[B_G64]);
-    changed += group_rounds<32, HashGroupResolve, true, E>(off, nbr, prev, next, gl.list[B_G32], 0, gl.len[B_G32]);
-    changed += group_rounds<16, HashGroupResolve, true, E>(off, nbr, prev, next, gl.list[B_G16], 0, gl.len[B_G16]);
-    changed += group_rounds<8, HashGroupResolve, true, E>(off, nbr, prev, next, gl.list[B_G8], 0, gl.len[B_G8]);
-    changed += group_rounds<4, HashGroupResolve, true, E>(off, nbr, prev, next, gl.list[B_G4], 0, gl.len[B_G4]);
-    block_count_add<4>(changed, counter, scratch);
-}
-
-template <typename E>
-static void launch_group_lists(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next) {
-    int64_t nv = 0;
-    double bytes = 0.0;
-    for (int b = B_G4; b <= B_G64; ++b) {
-        nv += w.cls[b].nv;
-        bytes += class_bytes<E>(w.cls[b]);   // all members: the graphs that stopped are not counted out
-    }
-    if (!nv) return;
-    GroupLists gl{};
-    for (int b = B_G4; b <= B_G64; ++b) gl.list[b] = g->batch->d_alist[b];
-    gl.len = g->batch->d_alen;
-    PendingLaunch pl;
-    prof_begin(g, "groups", bytes, &pl);
-    k_group_lists<E><<<grid_for(g, (nv + 63) / 64, 8), 256, 0, g->stream>>>((const E *)w.off, w.nbr, prev, next, gl,
-                                                                             g->d_counter, g->d_stop);
-    prof_end(g, &pl);
-}
-
 // Hub rows: tile (TT threads per 2048-edge tile), merge (TM threads per
 // (hub, partition) item), final.  The tile holds 48 KB of LDS per block and
 // the merge 68 KB with its slot list, 52 KB without (three or two blocks per
@@ -759,7 +721,7 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
     double hub_edges = 0.0;
     if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
     const int32_t *sel = learned_now(g) ? g->learn->d_hub_sel : nullptr;
-    uint32_t *hdist = g->learn ? g->learn->d_hub_distinct : nullptr;
+    uint32_t *hdist = distinct_out(g) ? g->learn->d_hub_distinct : nullptr;
     const int64_t *nsel = sel ? g->learn->d_plist_n + kCtrPlainHubs : nullptr;
     PendingLaunch pl;
     prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
@@ -783,7 +745,7 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
     prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);
     k_hub_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
         (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, sel ? sel + w.h0 : nullptr,
-        hdist ? hdist + w.h0 : nullptr, g->learn ? g->learn->distinct_out(g) : nullptr, prev, next, g->d_counter,
+        hdist ? hdist + w.h0 : nullptr, distinct_out(g), prev, next, g->d_counter,
         g->d_stop, nsel);
     prof_end(g, &pl);
 }
@@ -796,7 +758,6 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
 template <typename E>
 static void launch_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
     const int c = g->tune[b];
-    if (b <= B_G64 && batch_lists(g)) return;   // launch_group_lists runs the lane-group classes together
     switch (b) {
     case B_HUB:
         if (c == 1) launch_hubs<E, 512, 512, true>(g, w, prev, next);
@@ -873,7 +834,6 @@ static int hash_iteration(galp_graph *g, const IterView &w) {
         return GALP_OK;
     }
     for (int b = B_B4K; b >= B_G4; --b) launch_class<E>(g, w, b, prev, next);
-    if (batch_lists(g)) launch_group_lists<E>(g, w, prev, next);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(g->ctx, GALP_EHIP, std::string("hash kernel launch: ") + hipGetErrorString(e));
     return GALP_OK;
@@ -909,7 +869,6 @@ static int hash_iteration_incore(galp_graph *g) {
     hipStream_t ms = nullptr;
     if (rc == GALP_OK) rc = side_begin(g, 0, &ms);
     if (rc != GALP_OK) return rc;
-    if (batch_lists(g)) launch_group_lists<E>(g, w, prev, next);
     else
         for (int b = B_G64; b >= B_G4; --b) launch_class<E>(g, w, b, prev, next);
     rc = side_end(g, 0, ms);

@@ -1024,11 +1024,11 @@ static void sort_group(galp_graph *g, const IterView &w, int bid, const int32_t 
     if (g->d_vlist)
         k_group<G, SortGroupResolve, SortGroupResolve, true, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0,
                                                                   g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, g->d_counter, g->d_stop, nullptr, 0);
+            (const E *)w.off, w.nbr, prev, next, g->d_vlist, r.v0, r.nv, nullptr, g->d_counter, g->d_stop, nullptr, 0);
     else
         k_group<G, SortGroupResolve, SortGroupResolve, false, E><<<grid_for(g, blocks, blocks_per_cu), 256, 0,
                                                                    g->stream>>>(
-            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, g->d_counter, g->d_stop, nullptr, 0);
+            (const E *)w.off, w.nbr, prev, next, nullptr, r.v0, r.nv, nullptr, g->d_counter, g->d_stop, nullptr, 0);
     prof_end(g, &pl);
 }
 
