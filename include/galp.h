@@ -264,7 +264,9 @@ GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max,
  * kernels on one stream and on three and keeps the size rule's choice
  * unless the other is more than 3 % faster; under learned dispatch it then
  * judges the network against the plain classes per iteration
- * (galp_graph_set_learned_iterations).  It leaves the graph
+ * (galp_graph_set_learned_iterations) and, when some iterations go to the
+ * classes alone, times the class options again under that mask.  It leaves
+ * the graph
  * initialised (labels[i] = i) and the per-kernel profile totals as they
  * were.  cache_path (may be NULL) names a text file of earlier
  * choices keyed by device arch, method and the class's log2 sizes; hits are
@@ -277,8 +279,9 @@ GALP_API int galp_graph_autotune(galp_graph *g, int32_t reps, const char *cache_
 /* Learned dispatch in core: which iterations route by the network (bit i
  * for iteration i, bit 63 for every later one); the others run the plain
  * degree classes, the dispatcher the network is judged against.
- * galp_graph_autotune sets it by timing both per iteration (chosen[13] and
- * chosen[14] receive its low and high halves); the default routes every
+ * galp_graph_autotune sets it by timing both per iteration and keeping the
+ * network where it is more than 2 % faster (chosen[13] and chosen[14]
+ * receive the mask's low and high halves); the default routes every
  * iteration by the network.  Labels never depend on it.  No reference
  * counterpart. */
 GALP_API int galp_graph_set_learned_iterations(galp_graph *g, uint64_t mask);
