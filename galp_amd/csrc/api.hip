@@ -129,7 +129,7 @@ static void dev_free_all(galp_graph *g) {
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
                     g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
-                    g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts, g->d_err};
+                    g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts, g->d_err};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (g->h_pinned) (void)hipHostFree(g->h_pinned);
@@ -414,8 +414,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_tile_idx_off, g->nhub_tiles));
         GALP_TRY(dev_alloc(g, &g->d_tile_idx, ntile_idx));
         GALP_TRY(dev_alloc(g, &g->d_out_key, g->nhub_tiles * kHubTile));
-        GALP_TRY(dev_alloc(g, &g->d_out_cnt, g->nhub_tiles * kHubTile));
-        GALP_TRY(dev_alloc(g, &g->d_out_last, g->nhub_tiles * kHubTile));
+        GALP_TRY(dev_alloc(g, &g->d_out_ent, g->nhub_tiles * kHubTile));
         GALP_HIP(ctx, up(g->d_hub_tile0, hub_tile0.data(), 4 * hub_tile0.size()));
         GALP_HIP(ctx, up(g->d_hub_ntiles, hub_ntiles.data(), 4 * hub_ntiles.size()));
         GALP_HIP(ctx, up(g->d_tile_hub, tile_hub.data(), 4 * tile_hub.size()));

@@ -41,7 +41,16 @@ __host__ __device__ inline int bucket_of_deg(int64_t d) {
 extern const int64_t kBucketHi[NUM_BUCKETS];
 extern const char *const kBucketName[NUM_BUCKETS];
 
-constexpr int kHubTile = 2048;        // edges per hub tile (phase 1)
+constexpr int kHubTileBits = 11;
+constexpr int kHubTile = 1 << kHubTileBits;   // edges per hub tile (phase 1)
+// A hub tile's entry for one label: its count in the tile (<= kHubTile, 12
+// bits) and its last position within the tile, in one word; the merge adds
+// the tile's first row position back.
+__host__ __device__ constexpr uint32_t hub_entry(uint32_t count, uint32_t tile_pos) {
+    return count << kHubTileBits | tile_pos;
+}
+__host__ __device__ constexpr uint32_t hub_entry_count(uint32_t w) { return w >> kHubTileBits; }
+__host__ __device__ constexpr uint32_t hub_entry_pos(uint32_t w) { return w & (kHubTile - 1); }
 constexpr int kHubTileCapBits = 12;   // LDS table of a tile (4096 slots)
 constexpr int kHubPart = 2048;        // hub partitions: ceil(deg / kHubPart)
 constexpr int kHubMaxParts = 4096;    // so hub degree <= kHubPart * kHubMaxParts
@@ -262,7 +271,7 @@ struct galp_graph {
     int32_t *d_tile_hub = nullptr;          // hub of each tile
     int64_t *d_tile_idx_off = nullptr;      // offset of each tile's P_h + 1 partition offsets
     int32_t *d_tile_idx = nullptr;          // partition offsets of every tile
-    uint32_t *d_out_key = nullptr, *d_out_cnt = nullptr, *d_out_last = nullptr;  // kHubTile per tile
+    uint32_t *d_out_key = nullptr, *d_out_ent = nullptr;   // kHubTile per tile: label, hub_entry
     int32_t *d_item_hub = nullptr;          // hub of each merge item
     int32_t *d_item_part = nullptr;         // partition of each merge item
     unsigned long long *d_hub_best = nullptr;  // max (count << 32 | last) per hub

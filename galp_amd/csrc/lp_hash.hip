@@ -353,7 +353,7 @@ struct HubTilePolicy {
     const int64_t *tile_idx_off;
     int64_t ntiles;
     uint32_t *skey, *scnt, *slast, *hist;
-    uint32_t *out_key, *out_cnt, *out_last;
+    uint32_t *out_key, *out_ent;
     int32_t *tile_idx;
     int32_t *err;
     const int32_t *hub_sel;   // null: every hub; else only hubs with a non-zero flag
@@ -445,8 +445,7 @@ struct HubTilePolicy {
             if (creator[k]) {
                 const uint32_t q = atomicAdd(&hist[part[k]], 1u);
                 out_key[ob + q] = key[k];
-                out_cnt[ob + q] = cnt[k];
-                out_last[ob + q] = last[k];
+                out_ent[ob + q] = hub_entry(cnt[k], last[k] - it.row0);
             }
         __syncthreads();
         for (uint32_t p = threadIdx.x; p <= it.P; p += T) hist[p] = 0;
@@ -460,7 +459,7 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
            const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
            const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
            const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles,
-           uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last,
+           uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_ent,
            int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
            const int32_t *__restrict__ hub_sel, const int64_t *__restrict__ nsel,
            const int32_t *__restrict__ hub_hd) {
@@ -480,8 +479,8 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
     __syncthreads();
     HubTilePolicy<T, EPT, E, HD> pol{off,      tile_hub, hub_tile0,     hub_vid,  hub_nparts, tile_idx_off,
                                      ntiles,   tab,      tab + CAP,     tab + 2 * CAP,        tab + CAP,
-                                     out_key,  out_cnt,  out_last,      tile_idx, err,        hub_sel,
-                                     sscan,    hub_hd};
+                                     out_key,  out_ent,  tile_idx,      err,      hub_sel,    sscan,
+                                     hub_hd};
     pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
@@ -492,9 +491,8 @@ __global__ void __launch_bounds__(T)
 k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t nitems,
             const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
             const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
-            const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_cnt,
-            const uint32_t *__restrict__ out_last, unsigned long long *__restrict__ hub_best,
-            const int32_t *__restrict__ stop, int32_t *__restrict__ err,
+            const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
+            unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
             const int32_t *__restrict__ hub_sel, uint32_t *__restrict__ hub_distinct, const int64_t *__restrict__ nsel) {
     constexpr int CAP = 1 << kHubMergeCapBits;
     __shared__ uint32_t skey[CAP];
@@ -551,9 +549,10 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
                         else b = mid;
                     }
                     const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - shi[a]);
+                    const uint32_t w = out_ent[e];
                     sl = lds_insert<kHubMergeCapBits>(skey, out_key[e], &created, err);
-                    atomicAdd(&scnt[sl], out_cnt[e]);
-                    atomicMax(&slast[sl], out_last[e]);
+                    atomicAdd(&scnt[sl], hub_entry_count(w));
+                    atomicMax(&slast[sl], (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(w));
                 }
                 // one list append per wave
                 const unsigned long long ball = LIST ? __ballot(created) : 0ull;
@@ -728,18 +727,18 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
     // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
     k_hub_tile<TT, E, false><<<grid_for(g, nt, 3), TT, 0, g->stream>>>(
         (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
         g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr);
     if (sel)   // learned dispatch: the hubs predicted high-distinct
         k_hub_tile<TT, E, true><<<grid_for(g, nt, 3), TT, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-            g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+            g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
             g->d_err, sel, nsel, g->learn->d_hub_hd);
     prof_end(g, &pl);
     prof_begin(g, "hub_merge", 0.0, &pl);
     k_hub_merge<TM, LIST><<<grid_for(g, ni, TM == 256 || !LIST ? 3 : 2), TM, 0, g->stream>>>(
         g->d_item_hub + w.i0, g->d_item_part + w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
-        g->d_tile_idx, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, sel, hdist,
+        g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, sel, hdist,
         nsel);
     prof_end(g, &pl);
     prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);

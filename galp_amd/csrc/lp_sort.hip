@@ -667,7 +667,7 @@ struct HubSortTilePolicy {
     uint32_t *whist;      // waves x 256
     uint32_t *hist;       // kHubMaxParts + 1
     int *sscan;
-    uint32_t *out_key, *out_cnt, *out_last;
+    uint32_t *out_key, *out_ent;
     int32_t *tile_idx;
     int passes;
 
@@ -750,8 +750,7 @@ struct HubSortTilePolicy {
             if (run_end[k]) {
                 const uint32_t q = atomicAdd(&hist[part[k]], 1u);
                 out_key[ob + q] = key[k];
-                out_cnt[ob + q] = cnt[k];
-                out_last[ob + q] = last[k];
+                out_ent[ob + q] = hub_entry(cnt[k], last[k] - it.row0);
             }
         __syncthreads();
         for (uint32_t p = (uint32_t)tid; p <= it.P; p += T) hist[p] = 0;
@@ -764,8 +763,8 @@ k_hs_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int3
           const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
           const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
           const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles, uint32_t *__restrict__ out_key,
-          uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ out_last, int32_t *__restrict__ tile_idx,
-          const int32_t *__restrict__ stop, int passes) {
+          uint32_t *__restrict__ out_ent, int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop,
+          int passes) {
     constexpr int EPT = kHubTile / T;
     __shared__ uint32_t skey[2 * kHubTile];
     __shared__ uint16_t spos[2 * kHubTile];
@@ -774,9 +773,8 @@ k_hs_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int3
     __shared__ int sscan[T / kWave];
     if (*stop) return;
     for (int p = threadIdx.x; p <= kHubMaxParts; p += T) hist[p] = 0;
-    HubSortTilePolicy<T, E> pol{off,  tile_hub, hub_tile0, hub_vid, hub_nparts, tile_idx_off, ntiles,  skey,
-                                spos, whist,    hist,      sscan,   out_key,    out_cnt,      out_last, tile_idx,
-                                passes};
+    HubSortTilePolicy<T, E> pol{off,  tile_hub, hub_tile0, hub_vid, hub_nparts, tile_idx_off, ntiles, skey,
+                                spos, whist,    hist,      sscan,   out_key,    out_ent,      tile_idx, passes};
     pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
@@ -864,9 +862,9 @@ __global__ void __launch_bounds__(T)
 k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
            int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
            const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
-           const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_cnt,
-           const uint32_t *__restrict__ out_last, unsigned long long *__restrict__ hub_best,
-           const int32_t *__restrict__ stop, int32_t *__restrict__ err, int passes) {
+           const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
+           unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
+           int passes) {
     constexpr int N = kHsMergeCap;
     __shared__ uint32_t skey[2 * N];
     __shared__ uint16_t spos[2 * N];
@@ -920,9 +918,10 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
                         else b = mid;
                     }
                     const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - sstart[a]);
+                    const uint32_t w = out_ent[e];
                     skey[held + j] = out_key[e];
-                    scnt[held + j] = out_cnt[e];
-                    slast[held + j] = out_last[e];
+                    scnt[held + j] = hub_entry_count(w);
+                    slast[held + j] = (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(w);
                 }
                 held += take;
                 i0 += take;
@@ -1075,11 +1074,11 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
     prof_begin(g, "hub_sort", 8.0 * hub_edges, &pl);
     k_hs_tile<kHsTileT, E><<<grid_for(g, nt, tile_per_cu), kHsTileT, 0, g->stream>>>(
         (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_tile_idx, g->d_stop,
+        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
         g->sort->passes);
     k_hs_merge<kHsMergeT><<<grid_for(g, ni, merge_per_cu), kHsMergeT, 0, g->stream>>>(
         g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off, g->d_tile_idx,
-        g->d_out_key, g->d_out_cnt, g->d_out_last, g->d_hub_best, g->d_stop, g->d_err, g->sort->passes);
+        g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, g->sort->passes);
     k_hs_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
         (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop);
     prof_end(g, &pl);
