@@ -330,8 +330,10 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
                                            2 * esz * (n + 1)
                                      : 4 * m;
         int64_t need = resident + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
-        // the hub plan of both methods: merge items, tile outputs and partition offsets
-        need += 8 * g->nhub_items + (8 + 12 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
+        // the hub plan of both methods: merge items, tile outputs and partition
+        // offsets; the sort method's list of large merge items
+        need += 8 * g->nhub_items + (8 + 8 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
+        if (g->opts.method == GALP_METHOD_SORT) need += 8 * g->nhub_items + 8;
         if (g->reorder) need += 8 * n + (ooc ? 0 : 4 * m + 8 * (n + 1));  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;

@@ -185,6 +185,9 @@ struct LearnState {
 // labels < n.  Its hub path uses the hub plan both methods share.
 struct SortState {
     int passes = 0;
+    // hub merge items with more entries than one wave merges: their list
+    // and count (reset by the hub final kernel)
+    int64_t *d_big = nullptr, *d_nbig = nullptr;
 };
 
 

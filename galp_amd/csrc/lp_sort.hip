@@ -22,8 +22,9 @@
 //     search for each run's first element;
 //   - hubs (> 4096): each 2048-edge tile sorted in LDS and run-length encoded
 //     into (label, count, last) entries bucketed by label partition, then one
-//     block per (hub, partition) sorts that partition's entries of every tile
-//     and reduces each run; one 64-bit atomicMax of (count, last) per block.
+//     wave (at most 512 entries) or one block per (hub, partition) sorts that
+//     partition's entries of every tile and reduces each run; one 64-bit
+//     atomicMax of (count, last) per item.
 // The sort key is (label, position in the row), so every sort is stable in
 // the row order and the last element of a run holds the run's largest
 // position: the winner argmax (count, last) equals the hash method's.
@@ -317,7 +318,10 @@ struct RegBitonicPolicy {
         uint32_t before, after;   // thread tid - 1's last element, tid + 1's first
         if constexpr (NW == 1) {
             before = (uint32_t)__shfl_up((int)L[EPT - 1], 1);
-            after = lane == kWave - 1 ? kPad : (uint32_t)__shfl_down((int)L[0], 1);
+            // every lane shuffles (a shuffle under a condition reads 0 from
+            // lanes that skipped it), then the last lane takes the pad
+            const uint32_t dn = (uint32_t)__shfl_down((int)L[0], 1);
+            after = lane == kWave - 1 ? kPad : dn;
         } else {
             sedge[tid] = L[0];
             sedge[T + tid] = L[EPT - 1];
@@ -443,6 +447,7 @@ __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, i
     const int w = tid / kWave;
     const unsigned long long lt = (1ull << lane) - 1ull;
     int src = 0;
+#pragma unroll 1   // rolled loops: unrolled, they held 160 VGPRs in the hub tile kernel
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         const uint32_t *ks = skey + src * N;
@@ -452,6 +457,7 @@ __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, i
         for (int i = lane; i < 256; i += kWave) whist[w * 256 + i] = 0;
         __syncthreads();   // keys written; histogram rows clear
         // A: per-wave digit counts (the lowest lane of each peer group adds)
+#pragma unroll 1
         for (int c = 0; c < CH; ++c) {
             const int j = w * M + c * kWave + lane;
             const bool valid = j < len;
@@ -468,6 +474,7 @@ __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, i
 #pragma unroll
             for (int q = 0; q < 4; ++q) {   // lane owns digits 4*lane .. 4*lane+3
                 uint32_t t = 0;
+#pragma unroll 1
                 for (int ww = 0; ww < NW; ++ww) t += whist[ww * 256 + 4 * lane + q];
                 tot[q] = t;
                 run += t;
@@ -483,6 +490,7 @@ __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, i
             for (int q = 0; q < 4; ++q) {
                 const int d = 4 * lane + q;
                 uint32_t b2 = base;
+#pragma unroll 1
                 for (int ww = 0; ww < NW; ++ww) {
                     const uint32_t cnt = whist[ww * 256 + d];
                     whist[ww * 256 + d] = b2;
@@ -493,6 +501,7 @@ __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, i
         }
         __syncthreads();
         // B: stable scatter, each wave in chunk order with its running bases
+#pragma unroll 1
         for (int c = 0; c < CH; ++c) {
             if (w * M + c * kWave >= len) break;   // uniform per wave
             const int j = w * M + c * kWave + lane;
@@ -623,14 +632,17 @@ k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 // Hubs (degree > 4096), in the hub plan both methods share (tiles of
 // kHubTile edges, P_h = ceil(deg / kHubPart) label partitions per hub, merge
 // items (hub, partition)); every hub edge crosses HBM as a neighbour id and a
-// gathered label once, and the tiles' run-length output (12 bytes per
-// distinct label of a tile) once more each way.
+// gathered label once, and the tiles' run-length output (8 bytes per
+// distinct label of a tile: the label and a hub_entry word) once more each
+// way.
 //  1. tile (k_hs_tile): a block sorts one tile's (label, position) pairs by
 //     an LDS radix sort (stable, so equal labels keep row order), turns each
 //     run into (label, count, last position) and buckets the runs by label
 //     partition (counting sort in LDS), writing them with the P_h + 1
 //     partition offsets -- the same layout as the hash method's tiles;
-//  2. merge (k_hs_merge): a block per (hub, partition) stages that
+//  2. merge: items with at most kHsSmall entries are merged by one wave
+//     each (k_hs_merge_small), the others are listed for k_hs_merge: a block
+//     per (hub, partition) stages that
 //     partition's entries of every tile in LDS, tiles in row order, sorts
 //     them by label (stable radix sort, entry index as payload) and reduces
 //     each run: count = the sum of its counts (a block prefix sum), last =
@@ -856,11 +868,180 @@ __device__ int hs_merge_round(uint32_t *skey, uint16_t *spos, uint32_t *whist, u
     return *snruns;
 }
 
+// Small merge items: one wave per item (hub, partition) whose tiles hold at
+// most kHsSmall entries in it -- the usual case once labels have settled,
+// when a partition gets about one entry per tile.  Larger items go to a list
+// for the block merge (k_hs_merge).  The wave stages the item's entries in
+// its LDS slice (flat index = tile order), sorts (label << 32 | flat index)
+// with the register bitonic network, and reduces each run: its count is the
+// difference of the inclusive prefix sums of the counts at the run's end and
+// before its start (an exclusive max-scan carries the latter, prefix sums
+// being non-decreasing), its last position that of its last entry (the
+// stable order puts the latest tile last).
+constexpr int kHsSmall = 512;
+constexpr int kHsSmallT = 256;
+
+struct HsWaveLds {
+    uint32_t key[kHsSmall], cnt[kHsSmall], last[kHsSmall];
+    int32_t lo[kWave], start[kWave];
+};
+
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t x) {
+    const int lane = lane_id();
+    uint32_t inc = x;
+#pragma unroll
+    for (int sft = 1; sft < kWave; sft <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
+        if (lane >= sft) inc += y;
+    }
+    return inc - x;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_max(uint32_t x) {
+    const int lane = lane_id();
+    uint32_t inc = x;
+#pragma unroll
+    for (int sft = 1; sft < kWave; sft <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
+        if (lane >= sft) inc = inc > y ? inc : y;
+    }
+    const uint32_t ex = (uint32_t)__shfl_up((int)inc, 1);
+    return lane ? ex : 0u;
+}
+
+// max over the item's runs of (count << 32 | last); n <= 64 * EPT entries
+template <int EPT>
+__device__ unsigned long long hs_wave_reduce(const HsWaveLds &w, int n) {
+    const int lane = lane_id();
+    constexpr uint32_t kPad = 0xffffffffu;   // labels are < 2^31
+    unsigned long long key[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int i = lane * EPT + k;
+        key[k] = i < n ? (((unsigned long long)w.key[i] << 32) | (uint32_t)i) : ~0ull;
+    }
+    reg_bitonic<1, EPT>(key, lane, nullptr);
+    uint32_t L[EPT], P[EPT], slot[EPT];
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        L[k] = (uint32_t)(key[k] >> 32);
+        slot[k] = (uint32_t)key[k];
+        run += L[k] != kPad ? w.cnt[slot[k]] : 0u;
+        P[k] = run;   // inclusive, within the lane for now
+    }
+    const uint32_t base = wave_excl_sum(run);
+    // every lane shuffles: in `c ? shfl : x` only the lanes with c would, and
+    // a lane reading from one that skipped it gets 0
+    const uint32_t up = (uint32_t)__shfl_up((int)L[EPT - 1], 1);
+    const uint32_t dn = (uint32_t)__shfl_down((int)L[0], 1);
+    const uint32_t before = lane ? up : kPad;
+    const uint32_t after = lane < kWave - 1 ? dn : kPad;
+    // the prefix sum before each run start; their lane max is carried on
+    uint32_t lmax = 0;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        P[k] += base;
+        const uint32_t pv = k ? L[k - 1] : before;
+        const uint32_t c = L[k] != kPad ? w.cnt[slot[k]] : 0u;
+        if (L[k] != kPad && L[k] != pv) lmax = lmax > P[k] - c ? lmax : P[k] - c;
+    }
+    uint32_t cur = wave_excl_max(lmax);
+    unsigned long long best = 0;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        if (L[k] == kPad) continue;
+        const uint32_t pv = k ? L[k - 1] : before;
+        const uint32_t c = w.cnt[slot[k]];
+        if (L[k] != pv) cur = cur > P[k] - c ? cur : P[k] - c;
+        const uint32_t nx = k + 1 < EPT ? L[k + 1] : after;
+        if (L[k] != nx) {
+            const unsigned long long pk = ((unsigned long long)(P[k] - cur) << 32) | w.last[slot[k]];
+            best = pk > best ? pk : best;
+        }
+    }
+    return wave_max(best);
+}
+
+__global__ void __launch_bounds__(kHsSmallT)
+k_hs_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
+                 int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
+                 const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
+                 const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
+                 unsigned long long *__restrict__ hub_best, int64_t *__restrict__ big, int64_t *__restrict__ nbig,
+                 const int32_t *__restrict__ stop) {
+    constexpr int NW = kHsSmallT / kWave;
+    __shared__ HsWaveLds sw[NW];
+    if (*stop) return;
+    const int lane = lane_id();
+    HsWaveLds &w = sw[threadIdx.x / kWave];
+    const int64_t wid = (int64_t)blockIdx.x * NW + threadIdx.x / kWave;
+    for (int64_t it = ibase + wid; it < ibase + nitems; it += (int64_t)gridDim.x * NW) {
+        const int h = item_hub[it];
+        const int p = item_part[it];
+        const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        int total = 0;
+        for (int64_t c0 = t0; c0 < t0 + nt; c0 += kWave) {
+            const int64_t t = c0 + lane;
+            int len = 0;
+            if (t < t0 + nt) {
+                const int64_t ib = tile_idx_off[t];
+                len = tile_idx[ib + p + 1] - tile_idx[ib + p];
+            }
+            total += (int)wave_sum_u32((uint32_t)len);
+        }
+        if (total == 0) continue;
+        if (total > kHsSmall) {   // the block merge takes it
+            if (lane == 0) big[atomicAdd((unsigned long long *)nbig, 1ull)] = it;
+            continue;
+        }
+        int n = 0;
+        for (int64_t c0 = t0; c0 < t0 + nt; c0 += kWave) {
+            const int64_t t = c0 + lane;
+            int lo = 0, len = 0;
+            if (t < t0 + nt) {
+                const int64_t ib = tile_idx_off[t];
+                lo = tile_idx[ib + p];
+                len = tile_idx[ib + p + 1] - lo;
+            }
+            const int st = (int)wave_excl_sum((uint32_t)len);
+            const int ctot = __shfl(st + len, kWave - 1);
+            w.lo[lane] = lo;
+            w.start[lane] = st;
+            __builtin_amdgcn_wave_barrier();
+            const int nc = (int)min((int64_t)kWave, t0 + nt - c0);
+            for (int i = lane; i < ctot; i += kWave) {
+                int a = 0, b = nc;   // last tile with start <= i
+                while (b - a > 1) {
+                    const int mid = (a + b) >> 1;
+                    if (w.start[mid] <= i) a = mid;
+                    else b = mid;
+                }
+                const int64_t e = (c0 + a) * (int64_t)kHubTile + w.lo[a] + (i - w.start[a]);
+                const uint32_t ent = out_ent[e];
+                w.key[n + i] = out_key[e];
+                w.cnt[n + i] = hub_entry_count(ent);
+                w.last[n + i] = (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(ent);
+            }
+            n += ctot;
+            __builtin_amdgcn_wave_barrier();
+        }
+        unsigned long long best;
+        if (n <= kWave) best = hs_wave_reduce<1>(w, n);
+        else if (n <= 2 * kWave) best = hs_wave_reduce<2>(w, n);
+        else if (n <= 4 * kWave) best = hs_wave_reduce<4>(w, n);
+        else best = hs_wave_reduce<8>(w, n);
+        if (lane == 0 && best) atomicMax(&hub_best[h], best);
+        __builtin_amdgcn_wave_barrier();   // the slice is rewritten by the next item
+    }
+}
+
 // One block per merge item (hub, partition), grid-stride over the items.
 template <int T>
 __global__ void __launch_bounds__(T)
-k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
-           int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
+k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part,
+           const int64_t *__restrict__ big, const int64_t *__restrict__ nbig,
+           const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
            const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
            const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
            unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
@@ -876,7 +1057,9 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
     __shared__ unsigned long long sred64[T / kWave];
     if (*stop) return;
     const int tid = (int)threadIdx.x;
-    for (int64_t it = ibase + blockIdx.x; it < ibase + nitems; it += gridDim.x) {
+    const int64_t nitems = *nbig;
+    for (int64_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+        const int64_t it = big[k];
         const int h = item_hub[it];
         const int p = item_part[it];
         const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
@@ -948,8 +1131,10 @@ template <typename E>
 __global__ void __launch_bounds__(256)
 k_hs_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ hub_vid,
            int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ prev,
-           int32_t *__restrict__ next, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
+           int32_t *__restrict__ next, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
+           int64_t *__restrict__ nbig) {
     __shared__ uint32_t sred[4];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nbig = 0;   // the large-item list is used up
     if (*stop) return;
     uint32_t changed = 0;
     for (int64_t base = (int64_t)blockIdx.x * 256; base < nhubs; base += (int64_t)gridDim.x * 256) {
@@ -989,6 +1174,10 @@ static int salloc(galp_graph *g, T **p, int64_t count) {
 }
 
 void sort_free(galp_graph *g) {
+    if (g->sort) {
+        if (g->sort->d_big) (void)hipFree(g->sort->d_big);
+        if (g->sort->d_nbig) (void)hipFree(g->sort->d_nbig);
+    }
     delete g->sort;
     g->sort = nullptr;
 }
@@ -1002,7 +1191,10 @@ int sort_setup(galp_graph *g) {
     g->sort = S;
     const int bits = g->n > 1 ? 64 - __builtin_clzll((unsigned long long)(g->n - 1)) : 1;
     S->passes = (bits + 7) / 8;
-    return GALP_OK;
+    int rc = salloc(g, &S->d_big, g->nhub_items);
+    if (rc == GALP_OK) rc = salloc(g, &S->d_nbig, 1);
+    if (rc == GALP_OK) rc = dev_fill_u32(g, (uint32_t *)S->d_nbig, 2, 0u);
+    return rc;
 }
 
 template <typename E>
@@ -1076,11 +1268,17 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
         (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
         g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
         g->sort->passes);
+    const SortState *S = g->sort;
+    k_hs_merge_small<<<grid_for(g, (ni + kHsSmallT / kWave - 1) / (kHsSmallT / kWave), 6), kHsSmallT, 0,
+                       g->stream>>>(g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles,
+                                    g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best,
+                                    S->d_big, S->d_nbig, g->d_stop);
     k_hs_merge<kHsMergeT><<<grid_for(g, ni, merge_per_cu), kHsMergeT, 0, g->stream>>>(
-        g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off, g->d_tile_idx,
-        g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, g->sort->passes);
+        g->d_item_hub, g->d_item_part, S->d_big, S->d_nbig, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
+        g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, S->passes);
     k_hs_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
-        (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop);
+        (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop,
+        S->d_nbig);
     prof_end(g, &pl);
     return GALP_OK;
 }
