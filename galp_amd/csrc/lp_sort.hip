@@ -434,8 +434,11 @@ __device__ __forceinline__ unsigned long long digit_peers8(uint32_t d, bool vali
 // every wave owns the contiguous elements [w*M, (w+1)*M) (M = N / waves),
 // 8 ballots give each lane the lanes of its 64-element chunk with the same
 // digit, per-wave digit histograms (whist: waves x 256) and one scan over
-// (digit, wave) make each pass stable across the block.  Returns the half
-// (0 or 1) that holds the sorted keys; ends with a barrier.
+// (digit, wave) make each pass stable across the block.  `passes` is the
+// cap; the pass count is that of the largest key present (digits above it
+// are 0 everywhere, and a pass over them would keep the order): converged
+// rows hold mostly small labels after the degree-descending renumbering.
+// Returns the half (0 or 1) that holds the sorted keys; ends with a barrier.
 template <int T, int N>
 __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, int len, int passes) {
     constexpr int NW = T / kWave;
@@ -446,6 +449,19 @@ __device__ int lds_radix_sort(uint32_t *skey, uint16_t *spos, uint32_t *whist, i
     const int lane = lane_id();
     const int w = tid / kWave;
     const unsigned long long lt = (1ull << lane) - 1ull;
+    {
+        __syncthreads();   // the caller's keys are in place
+        uint32_t mx = 0;
+        for (int j = tid; j < len; j += T) mx = mx > skey[j] ? mx : skey[j];
+        mx = wave_max(mx);
+        if (lane == 0) whist[w * 256] = mx;
+        __syncthreads();
+#pragma unroll 1
+        for (int ww = 0; ww < NW; ++ww) mx = mx > whist[ww * 256] ? mx : whist[ww * 256];
+        const int need = mx ? (32 - __clz((int)mx) + 7) / 8 : 0;
+        passes = need < passes ? need : passes;
+        __syncthreads();   // read before the first pass clears the rows
+    }
     int src = 0;
 #pragma unroll 1   // rolled loops: unrolled, they held 160 VGPRs in the hub tile kernel
     for (int p = 0; p < passes; ++p) {
@@ -1257,7 +1273,7 @@ static void sort_reg(galp_graph *g, const IterView &w, int bid, const int32_t *p
 // [i0, i1): tile sorts, partition merges, winners.  tile_per_cu and
 // merge_per_cu size the grids (the autotuner's options).
 template <typename E>
-static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next, int tile_per_cu = 3,
+static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int32_t *next, int tile_per_cu = 2,
                      int merge_per_cu = 1) {
     const int64_t nt = w.t1 - w.t0, ni = w.i1 - w.i0, nh = w.h1 - w.h0;
     double hub_edges = 0.0;
@@ -1287,14 +1303,14 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
 // 0 is the default, 1 and 2 the alternatives the autotuner times --
 // workgroup size x keys per thread, the radix sort or the bitonic network,
 // blocks per CU; for lane groups the grid; for hubs the blocks per CU of the
-// tile grid (3 or 2) and of the merge grid (1 or 2).  Every option computes
+// tile grid (2 or 3) and of the merge grid (1 or 2).  Every option computes
 // the same labels.
 template <typename E>
 static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
     const int c = g->tune[b];
     switch (b) {
     case B_HUB:
-        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 2 : 3, c == 2 ? 2 : 1);
+        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 3 : 2, c == 2 ? 2 : 1);
         break;
     case B_B4K:
         if (c == 1) sort_reg<4, 16, E>(g, w, b, prev, next, 2);
