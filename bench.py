@@ -93,13 +93,17 @@ def kernel_source_hash():
     return h.hexdigest()[:16]
 
 
-def measured_traffic(workload, variant, kernel, source_hash=None):
+def measured_traffic(workload, variant, kernel, source_hash=None, alg_bytes=None):
     """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE counters,
     each scaled by what it reads for a copy of known size), measured for this
     workload and variant on kernel sources whose hash is source_hash (the
-    current sources when None).  Returns (bytes or None, stale): stale is True
-    when an entry exists but was measured on other kernel sources, and the
-    figure is then withheld."""
+    current sources when None).  alg_bytes: this run's algorithmic bytes per
+    launch; the measured bytes are then rescaled by it (bytes per algorithmic
+    byte of the measured launches), because a run's mix of launches -- how
+    many iterations the learned dispatcher routes to the kernel, and with
+    how much work -- can differ from the counter run's.  Returns (bytes or
+    None, stale): stale is True when an entry exists but was measured on
+    other kernel sources, and the figure is then withheld."""
     try:
         with open(TRAFFIC_FILE) as f:
             entries = json.load(f).get("entries", [])
@@ -110,7 +114,10 @@ def measured_traffic(workload, variant, kernel, source_hash=None):
     for e in entries:
         if e.get("workload") == workload and e.get("variant") == variant and e.get("kernel") == kernel:
             if e.get("source_hash") == want:
-                return float(e["bytes_per_launch"]), False
+                b = float(e["bytes_per_launch"])
+                if alg_bytes:
+                    b *= float(alg_bytes) / float(e["algorithmic_bytes_per_launch"])
+                return b, False
             stale = True
     return None, stale
 
@@ -205,11 +212,27 @@ def time_batched(ctx, count, nmax, check, steps, niter):
                         f"loop, per-graph stop rule, stopped graphs dropped from the work lists"}
 
 
-def time_variant(ctx, off, nbr, steps, warmup, niter, mlp=None, tune=False, **opts):
+def parse_mask(s):
+    """The learned-iteration mask from its printed form: character i is
+    iteration i + 1 ('1': routed by the network); the last one holds for
+    every later iteration."""
+    mask = 0
+    for i, c in enumerate(s):
+        if c == "1":
+            mask |= 1 << i
+    if s and s[-1] == "1":
+        mask |= ((1 << 64) - 1) ^ ((1 << len(s)) - 1)
+    return mask
+
+
+def time_variant(ctx, off, nbr, steps, warmup, niter, mlp=None, tune=False, mask=None, **opts):
     """Times `steps` runs without instrumentation (the value), then the same
     runs again with a HIP event pair around every kernel launch (the
     per-kernel roofline; events cost a few percent of wall time).  tune:
-    autotune the class kernels first (set-up, outside the timed region)."""
+    autotune the class kernels first (set-up, outside the timed region).
+    mask: the learned-iteration mask to use instead of the autotuner's (the
+    counter runs, where profiling skews the autotuner's timings, pass the
+    one a plain run chose so that both launch the same kernels)."""
     from galp_amd import _abi
     out = {}
     choice = None
@@ -224,6 +247,9 @@ def time_variant(ctx, off, nbr, steps, warmup, niter, mlp=None, tune=False, **op
                     out["tuned_streams"] = g.tuned_streams
                     if mlp is not None:   # the judge's per-iteration choice of router
                         out["learned_iterations"] = g.learned_iterations
+                        if mask is not None:
+                            out["learned_iterations"] = mask
+                            g.set_learned_iterations(mask)
                 else:
                     g.set_tuning(choice)
                     if mlp is not None:
@@ -337,6 +363,9 @@ def main():
                     help="only the batched measurement (its GTEPS is then the value)")
     ap.add_argument("--save-mlp", help="write the trained dispatcher network (float32 .npy)")
     ap.add_argument("--load-mlp", help="use this network instead of training one (float32 .npy)")
+    ap.add_argument("--learned-mask", metavar="BITS",
+                    help="learned-iteration mask as printed in learned_iterations (e.g. 1100111111), used instead of "
+                         "the autotuner's judge (counter runs)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only copy MIB MiB device-to-device (a reference kernel for the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -418,7 +447,8 @@ def main():
     variants = {}
     for name in names:
         r = time_variant(ctx, off, nbr, args.steps, args.warmup, args.iters,
-                         mlp=mlp if name.startswith("hash_learned") else None, **table[name])
+                         mlp=mlp if name.startswith("hash_learned") else None,
+                         mask=parse_mask(args.learned_mask) if args.learned_mask else None, **table[name])
         gteps = m * r["iters"] / r["wall_s"] / 1e9
         variants[name] = r
         log(f"bench: {name}: {r['iters']} iterations in {r['wall_s'] * 1e3:.2f} ms wall "
@@ -451,7 +481,7 @@ def main():
     dom_bytes = dom["alg_bytes"] / dom["launches"]
     achieved = dom_bytes / dom_avg_s / 1e9
     src_hash = kernel_source_hash()
-    traffic, traffic_stale = measured_traffic(workload, head, dom["name"], src_hash)
+    traffic, traffic_stale = measured_traffic(workload, head, dom["name"], src_hash, dom_bytes)
     vg = {k: round(m * v["iters"] / v["wall_s"] / 1e9, 3) for k, v in variants.items()}
     # out-of-core: the neighbour array crosses the host link once per iteration
     link = {k: round(4.0 * m * v["iters"] / v["wall_s"] / 1e9, 2) for k, v in variants.items() if "_ooc_" in k}
@@ -477,9 +507,11 @@ def main():
                    "iterations_per_step": iters // args.steps},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "alg_bytes_per_launch": dom_bytes,
-                     "traffic_source": "lookup: profiles/hbm_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
-                                       "entries keyed by workload, variant, kernel and kernel-source hash)",
+                     "traffic": round(traffic) if traffic else None, "alg_bytes_per_launch": dom_bytes,
+                     "traffic_source": "profiles/hbm_traffic.json: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE bytes per "
+                                       "algorithmic byte of the kernel's launches in the counter run, times this "
+                                       "run's algorithmic bytes per launch; entries keyed by workload, variant, "
+                                       "kernel and kernel-source hash",
                      "kernel_source_hash": src_hash, "traffic_stale": traffic_stale,
                      "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "iteration_roofline": {"alg_bytes": b_iter, "achieved_GBps": round(b_iter / per_iter_s / 1e9, 1),

@@ -301,7 +301,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
             return fail(ctx, GALP_EUNSUPPORTED, "a vertex of degree " + std::to_string(d) +
                                                     " exceeds the hub path's limit of " +
                                                     std::to_string((int64_t)kHubPart * kHubMaxParts));
-        const int32_t P = (int32_t)((d + kHubPart - 1) / kHubPart);
+        const int64_t part = g->opts.method == GALP_METHOD_SORT ? kHubPartSort : kHubPart;
+        const int32_t P = (int32_t)std::min<int64_t>(kHubMaxParts, (d + part - 1) / part);
         const int32_t nt = (int32_t)((d + kHubTile - 1) / kHubTile);
         hub_nparts.push_back(P);
         hub_tile0.push_back((int32_t)tile_hub.size());

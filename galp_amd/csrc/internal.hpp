@@ -54,6 +54,10 @@ __host__ __device__ constexpr uint32_t hub_entry_pos(uint32_t w) { return w & (k
 constexpr int kHubTileCapBits = 12;   // LDS table of a tile (4096 slots)
 constexpr int kHubPart = 2048;        // hub partitions: ceil(deg / kHubPart)
 constexpr int kHubMaxParts = 4096;    // so hub degree <= kHubPart * kHubMaxParts
+// The sort method cuts hubs finer, min(kHubMaxParts, ceil(deg / 512)): a
+// partition then holds about 512 entries even when every label of the row
+// is distinct (the first iterations), which one wave merges in registers.
+constexpr int kHubPartSort = 512;
 constexpr int kHubMergeCapBits = 12;  // LDS table of one partition (4096 slots)
 
 // Parts of the neighbour-id space: (u >> kPartLineBits) mod kNbrParts, the

@@ -646,16 +646,17 @@ k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 
 // ---------------------------------------------------------------------------
 // Hubs (degree > 4096), in the hub plan both methods share (tiles of
-// kHubTile edges, P_h = ceil(deg / kHubPart) label partitions per hub, merge
-// items (hub, partition)); every hub edge crosses HBM as a neighbour id and a
+// kHubTile edges, P_h label partitions per hub -- for this method
+// min(kHubMaxParts, ceil(deg / kHubPartSort)) -- and merge items (hub,
+// partition)); every hub edge crosses HBM as a neighbour id and a
 // gathered label once, and the tiles' run-length output (8 bytes per
 // distinct label of a tile: the label and a hub_entry word) once more each
 // way.
-//  1. tile (k_hs_tile): a block sorts one tile's (label, position) pairs by
-//     an LDS radix sort (stable, so equal labels keep row order), turns each
-//     run into (label, count, last position) and buckets the runs by label
-//     partition (counting sort in LDS), writing them with the P_h + 1
-//     partition offsets -- the same layout as the hash method's tiles;
+//  1. tile (k_hs_tile_wave): one wave sorts one tile's (partition, label,
+//     position) keys in registers, turns each run into (label, count, last
+//     position) -- the runs come out bucketed by label partition -- and
+//     writes them with the P_h + 1 partition offsets, the same layout as
+//     the hash method's tiles;
 //  2. merge: items with at most kHsSmall entries are merged by one wave
 //     each (k_hs_merge_small), the others are listed for k_hs_merge: a block
 //     per (hub, partition) stages that
@@ -668,142 +669,143 @@ k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 //     the front and the next entries are appended behind them.  One 64-bit
 //     atomicMax of (count << 32 | last) per block picks the hub's winner;
 //  3. final (k_hs_final): the winner is the label at position `last`.
-constexpr int kHsTileT = 512;
-constexpr int kHsTileEPT = kHubTile / kHsTileT;   // 4
 constexpr int kHsMergeT = 1024;
 constexpr int kHsMergeCap = 4096;                 // entries staged per round
-static_assert(kHubTile == 2048 && kHsTileEPT == 4, "hub tile layout");
 
-template <int T, typename E>
-struct HubSortTilePolicy {
-    static constexpr int EPT = kHubTile / T;
-    struct Item {
-        bool valid;
-        int len;
-        int64_t e0;       // absolute index of the tile's first edge
-        uint32_t row0;    // position of that edge in its row
-        uint32_t P;       // partitions of the hub
-        int64_t idx;      // offset of the tile's partition offsets
-        int64_t t;        // tile id
-    };
-    const E *off;
-    const int32_t *tile_hub, *hub_tile0, *hub_vid, *hub_nparts;
-    const int64_t *tile_idx_off;
-    int64_t ntiles;
-    uint32_t *skey;       // 2 x kHubTile
-    uint16_t *spos;       // 2 x kHubTile
-    uint32_t *whist;      // waves x 256
-    uint32_t *hist;       // kHubMaxParts + 1
-    int *sscan;
-    uint32_t *out_key, *out_ent;
-    int32_t *tile_idx;
-    int passes;
-
-    __device__ Item item(int64_t t) const {
-        Item it{false, 0, 0, 0, 1, 0, t};
-        if (t < ntiles) {
-            const int h = tile_hub[t];
-            const int64_t v = hub_vid[h];
-            const int64_t rb = (int64_t)off[v], re = (int64_t)off[v + 1];
-            it.row0 = (uint32_t)((t - (int64_t)hub_tile0[h]) * (int64_t)kHubTile);   // tiles of a hub are consecutive
-            it.e0 = rb + it.row0;
-            it.len = (int)min((int64_t)kHubTile, re - it.e0);
-            it.P = (uint32_t)hub_nparts[h];
-            it.idx = tile_idx_off[t];
-            it.valid = true;
-        }
-        return it;
+// Wave-wide exclusive scans (sum, max) of one value per lane.
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t x) {
+    const int lane = lane_id();
+    uint32_t inc = x;
+#pragma unroll
+    for (int sft = 1; sft < kWave; sft <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
+        if (lane >= sft) inc += y;
     }
-    __device__ int64_t edge0(const Item &it) const { return it.e0; }
+    return inc - x;
+}
 
-    __device__ void process(const Item &it, const int32_t (&lab)[EPT]) {
-        const int len = it.len;
-        const int tid = (int)threadIdx.x;
-        __syncthreads();   // the previous tile's readers are done
+__device__ __forceinline__ uint32_t wave_excl_max(uint32_t x) {
+    const int lane = lane_id();
+    uint32_t inc = x;
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int pos = k * T + tid;
-            if (pos < len) {
-                skey[pos] = (uint32_t)lab[k];
-                spos[pos] = (uint16_t)pos;
-            }
-        }
-        const int src = lds_radix_sort<T, kHubTile>(skey, spos, whist, len, passes);
-        const uint32_t *kf = skey + src * kHubTile;
-        const uint16_t *pf = spos + src * kHubTile;
-        // runs: the thread holding a run's end finds its start by binary search
-        uint32_t key[EPT], cnt[EPT], last[EPT], part[EPT];
-        bool run_end[EPT];
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int q = k * T + tid;
-            run_end[k] = false;
-            if (q < len) {
-                key[k] = kf[q];
-                if (q == len - 1 || kf[q + 1] != key[k]) {
-                    int lo = 0, hi = q;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (kf[mid] < key[k]) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    run_end[k] = true;
-                    cnt[k] = (uint32_t)(q - lo + 1);
-                    last[k] = it.row0 + (uint32_t)pf[q];   // stable: the run's largest position
-                    part[k] = hub_partition(key[k], it.P);
-                    atomicAdd(&hist[part[k] + 1], 1u);
-                }
-            }
-        }
-        __syncthreads();
-        // hist[p] := first slot of partition p (inclusive scan of the counts
-        // at p + 1), which are also the tile's partition offsets
-        {
-            const uint32_t np = it.P + 1;
-            const uint32_t per = (np + T - 1) / T;
-            const uint32_t p0 = (uint32_t)tid * per;
-            int run = 0;
-            for (uint32_t p = p0; p < p0 + per && p < np; ++p) run += (int)hist[p];
-            int acc = block_exclusive_scan<T / kWave>(run, sscan);   // barriers inside
-            for (uint32_t p = p0; p < p0 + per && p < np; ++p) {
-                acc += (int)hist[p];
-                hist[p] = (uint32_t)acc;
-                tile_idx[it.idx + p] = acc;
-            }
-        }
-        __syncthreads();
-        const int64_t ob = it.t * (int64_t)kHubTile;
-#pragma unroll
-        for (int k = 0; k < EPT; ++k)
-            if (run_end[k]) {
-                const uint32_t q = atomicAdd(&hist[part[k]], 1u);
-                out_key[ob + q] = key[k];
-                out_ent[ob + q] = hub_entry(cnt[k], last[k] - it.row0);
-            }
-        __syncthreads();
-        for (uint32_t p = (uint32_t)tid; p <= it.P; p += T) hist[p] = 0;
+    for (int sft = 1; sft < kWave; sft <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
+        if (lane >= sft) inc = inc > y ? inc : y;
     }
-};
+    const uint32_t ex = (uint32_t)__shfl_up((int)inc, 1);
+    return lane ? ex : 0u;
+}
 
-template <int T, typename E>
-__global__ void __launch_bounds__(T)
-k_hs_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
-          const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
-          const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
-          const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles, uint32_t *__restrict__ out_key,
-          uint32_t *__restrict__ out_ent, int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop,
-          int passes) {
-    constexpr int EPT = kHubTile / T;
-    __shared__ uint32_t skey[2 * kHubTile];
-    __shared__ uint16_t spos[2 * kHubTile];
-    __shared__ uint32_t whist[(T / kWave) * 256];
-    __shared__ uint32_t hist[kHubMaxParts + 1];
-    __shared__ int sscan[T / kWave];
+// The tile phase with one wave per tile and no LDS: the wave sorts the
+// tile's keys (partition << 43 | label << 11 | position in the tile, padded
+// with all-ones) with the register bitonic network, 32 per lane, so runs of
+// one label come out grouped by label partition, partitions in order; a run
+// end gives the entry (label, hub_entry(run length, its last position)),
+// written at its rank among the tile's run ends, and the partition offsets
+// are the ranks where the partition changes.  Waves share nothing, so there
+// is no barrier and no LDS.  (A block-wide LDS radix sort of the tile, four
+// 8-bit passes of three barriers each plus a counting sort by partition,
+// took 34 ms per iteration at RMAT-26 where this takes a fraction.)
+constexpr int kHsWaveEPT = kHubTile / kWave;   // 32
+constexpr int kHsWaveT = 256;
+
+__device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v) {
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, 1);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), 1);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long shfl_down_u64(unsigned long long v) {
+    const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)v, 1);
+    const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(v >> 32), 1);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+template <typename E>
+__global__ void __launch_bounds__(kHsWaveT)
+k_hs_tile_wave(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ prev,
+               const int32_t *__restrict__ tile_hub, const int32_t *__restrict__ hub_tile0,
+               const int32_t *__restrict__ hub_vid, const int32_t *__restrict__ hub_nparts,
+               const int64_t *__restrict__ tile_idx_off, int64_t tbase, int64_t ntiles,
+               uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_ent, int32_t *__restrict__ tile_idx,
+               const int32_t *__restrict__ stop) {
+    constexpr int EPT = kHsWaveEPT;
+    constexpr int NW = kHsWaveT / kWave;
+    static_assert(kHubMaxParts <= (1 << 12) && kHubTile == (1 << 11), "key layout: 12 + 31 + 11 bits");
     if (*stop) return;
-    for (int p = threadIdx.x; p <= kHubMaxParts; p += T) hist[p] = 0;
-    HubSortTilePolicy<T, E> pol{off,  tile_hub, hub_tile0, hub_vid, hub_nparts, tile_idx_off, ntiles, skey,
-                                spos, whist,    hist,      sscan,   out_key,    out_ent,      tile_idx, passes};
-    pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
+    const int lane = lane_id();
+    const int64_t wid = (int64_t)blockIdx.x * NW + threadIdx.x / kWave;
+    for (int64_t t = tbase + wid; t < ntiles; t += (int64_t)gridDim.x * NW) {
+        const int h = tile_hub[t];
+        const int64_t v = hub_vid[h];
+        const int64_t rb = (int64_t)off[v], re = (int64_t)off[v + 1];
+        const int64_t row0 = (t - (int64_t)hub_tile0[h]) * (int64_t)kHubTile;
+        const int64_t e0 = rb + row0;
+        const int len = (int)min((int64_t)kHubTile, re - e0);
+        const uint32_t P = (uint32_t)hub_nparts[h];
+        unsigned long long key[EPT];
+        {
+            int32_t u[EPT];
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                const int pos = k * kWave + lane;
+                u[k] = __builtin_nontemporal_load(&nbr[pos < len ? e0 + pos : e0]);
+            }
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                const int pos = k * kWave + lane;
+                const uint32_t L = (uint32_t)prev[u[k]];
+                key[k] = pos < len ? ((unsigned long long)hub_partition(L, P) << 43 |
+                                      (unsigned long long)L << 11 | (unsigned long long)pos)
+                                   : ~0ull;
+            }
+        }
+        reg_bitonic<1, EPT>(key, lane, nullptr);
+        // element e = lane * EPT + k; g = partition and label (key >> 11)
+        const unsigned long long before = shfl_up_u64(key[EPT - 1]);
+        const unsigned long long after = shfl_down_u64(key[0]);
+        int last_start = -1;   // this lane's last run start
+        uint32_t nend = 0;     // run ends in this lane
+        int maxpart = -1;      // partition of this lane's last run end
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const unsigned long long g = key[k] >> 11;
+            if (key[k] == ~0ull) continue;
+            const unsigned long long gp = (k ? key[k - 1] : before) >> 11;
+            const unsigned long long gn = (k + 1 < EPT ? key[k + 1] : after) >> 11;
+            if ((lane == 0 && k == 0) || g != gp) last_start = lane * EPT + k;
+            if ((lane == kWave - 1 && k == EPT - 1) || g != gn) {
+                ++nend;
+                maxpart = (int)(key[k] >> 43);
+            }
+        }
+        // carried from the lanes before: their last run start, the number of
+        // run ends and the partition of the last one
+        int start = (int)wave_excl_max((uint32_t)(last_start + 1)) - 1;
+        uint32_t q = wave_excl_sum(nend);
+        int pp = (int)wave_excl_max((uint32_t)(maxpart + 1)) - 1;
+        const uint32_t total = (uint32_t)__shfl((int)(q + nend), kWave - 1);
+        const int64_t ob = t * (int64_t)kHubTile, idx = tile_idx_off[t];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            if (key[k] == ~0ull) continue;
+            const unsigned long long g = key[k] >> 11;
+            const unsigned long long gp = (k ? key[k - 1] : before) >> 11;
+            const unsigned long long gn = (k + 1 < EPT ? key[k + 1] : after) >> 11;
+            const int e = lane * EPT + k;
+            if ((lane == 0 && k == 0) || g != gp) start = e;
+            if ((lane == kWave - 1 && k == EPT - 1) || g != gn) {
+                const int pt = (int)(key[k] >> 43);
+                for (int p = pp + 1; p <= pt; ++p) tile_idx[idx + p] = (int32_t)q;   // partitions starting here
+                pp = pt;
+                out_key[ob + q] = (uint32_t)(g & 0xffffffffull);
+                out_ent[ob + q] = hub_entry((uint32_t)(e - start + 1), (uint32_t)(key[k] & (kHubTile - 1)));
+                ++q;
+            }
+        }
+        // the partitions after the last run end start at the tile's end
+        const int last_part = (int)wave_max((uint32_t)(maxpart + 1)) - 1;
+        for (int p = last_part + 1 + lane; p <= (int)P; p += kWave) tile_idx[idx + p] = (int32_t)total;
+    }
 }
 
 // Block inclusive prefix sum of x[0, len) into y[0, len) (len <= T * EPT):
@@ -901,29 +903,6 @@ struct HsWaveLds {
     uint32_t key[kHsSmall], cnt[kHsSmall], last[kHsSmall];
     int32_t lo[kWave], start[kWave];
 };
-
-__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t x) {
-    const int lane = lane_id();
-    uint32_t inc = x;
-#pragma unroll
-    for (int sft = 1; sft < kWave; sft <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
-        if (lane >= sft) inc += y;
-    }
-    return inc - x;
-}
-
-__device__ __forceinline__ uint32_t wave_excl_max(uint32_t x) {
-    const int lane = lane_id();
-    uint32_t inc = x;
-#pragma unroll
-    for (int sft = 1; sft < kWave; sft <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)inc, sft);
-        if (lane >= sft) inc = inc > y ? inc : y;
-    }
-    const uint32_t ex = (uint32_t)__shfl_up((int)inc, 1);
-    return lane ? ex : 0u;
-}
 
 // max over the item's runs of (count << 32 | last); n <= 64 * EPT entries
 template <int EPT>
@@ -1280,10 +1259,10 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
     if (w.h0 == 0 && w.h1 == g->nhubs) hub_edges = (double)g->bucket[B_HUB].edges;
     PendingLaunch pl;
     prof_begin(g, "hub_sort", 8.0 * hub_edges, &pl);
-    k_hs_tile<kHsTileT, E><<<grid_for(g, nt, tile_per_cu), kHsTileT, 0, g->stream>>>(
-        (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
-        g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
-        g->sort->passes);
+    k_hs_tile_wave<E><<<grid_for(g, (nt + kHsWaveT / kWave - 1) / (kHsWaveT / kWave), tile_per_cu), kHsWaveT, 0,
+                        g->stream>>>((const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid,
+                                     g->d_hub_nparts, g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent,
+                                     g->d_tile_idx, g->d_stop);
     const SortState *S = g->sort;
     k_hs_merge_small<<<grid_for(g, (ni + kHsSmallT / kWave - 1) / (kHsSmallT / kWave), 6), kHsSmallT, 0,
                        g->stream>>>(g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles,
@@ -1303,14 +1282,15 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
 // 0 is the default, 1 and 2 the alternatives the autotuner times --
 // workgroup size x keys per thread, the radix sort or the bitonic network,
 // blocks per CU; for lane groups the grid; for hubs the blocks per CU of the
-// tile grid (2 or 3) and of the merge grid (1 or 2).  Every option computes
+// tile grid (2 or 1 four-wave blocks; at 224 VGPRs two fit) and of the
+// merge grid (1 or 2).  Every option computes
 // the same labels.
 template <typename E>
 static int sort_class(galp_graph *g, const IterView &w, int b, const int32_t *prev, int32_t *next) {
     const int c = g->tune[b];
     switch (b) {
     case B_HUB:
-        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 3 : 2, c == 2 ? 2 : 1);
+        if (w.h1 > w.h0) return sort_hubs<E>(g, w, prev, next, c == 1 ? 1 : 2, c == 2 ? 2 : 1);
         break;
     case B_B4K:
         if (c == 1) sort_reg<4, 16, E>(g, w, b, prev, next, 2);

@@ -29,6 +29,8 @@ def test_measured_traffic_lookup(tmp_path, monkeypatch):
     f.write_text(json.dumps({"entries": [entry]}))
     monkeypatch.setattr(bench, "TRAFFIC_FILE", str(f))
     assert bench.measured_traffic(RMAT22, "hash_learned_tuned", "cand", "abc") == (900.0, False)
+    # rescaled to a run whose launches carry twice the algorithmic bytes
+    assert bench.measured_traffic(RMAT22, "hash_learned_tuned", "cand", "abc", 1200) == (1800.0, False)
     # measured on other kernel sources: withheld and flagged
     assert bench.measured_traffic(RMAT22, "hash_learned_tuned", "cand", "def") == (None, True)
     assert bench.measured_traffic(RMAT22, "hash_plain", "no-such-kernel", "abc") == (None, False)
