@@ -22,7 +22,7 @@
 //     search for each run's first element;
 //   - hubs (> 4096): each 2048-edge tile sorted in LDS and run-length encoded
 //     into (label, count, last) entries bucketed by label partition, then one
-//     wave (at most 512 entries) or one block per (hub, partition) sorts that
+//     wave (at most 1024 entries) or one block per (hub, partition) sorts that
 //     partition's entries of every tile and reduces each run; one 64-bit
 //     atomicMax of (count, last) per item.
 // The sort key is (label, position in the row), so every sort is stable in
@@ -763,21 +763,24 @@ k_hs_tile_wave(const E *__restrict__ off, const int32_t *__restrict__ nbr, const
         // element e = lane * EPT + k; g = partition and label (key >> 11)
         const unsigned long long before = shfl_up_u64(key[EPT - 1]);
         const unsigned long long after = shfl_down_u64(key[0]);
-        int last_start = -1;   // this lane's last run start
-        uint32_t nend = 0;     // run ends in this lane
-        int maxpart = -1;      // partition of this lane's last run end
+        // run starts and ends of this lane's elements as bit masks (bit k:
+        // element k), so the write loop below keeps no 64-bit neighbours
+        uint32_t smask = 0, emask = 0;
+        int maxpart = -1;   // partition of this lane's last run end
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const unsigned long long g = key[k] >> 11;
             if (key[k] == ~0ull) continue;
+            const unsigned long long g = key[k] >> 11;
             const unsigned long long gp = (k ? key[k - 1] : before) >> 11;
             const unsigned long long gn = (k + 1 < EPT ? key[k + 1] : after) >> 11;
-            if ((lane == 0 && k == 0) || g != gp) last_start = lane * EPT + k;
+            if ((lane == 0 && k == 0) || g != gp) smask |= 1u << k;
             if ((lane == kWave - 1 && k == EPT - 1) || g != gn) {
-                ++nend;
+                emask |= 1u << k;
                 maxpart = (int)(key[k] >> 43);
             }
         }
+        const int last_start = smask ? lane * EPT + 31 - __clz((int)smask) : -1;
+        const uint32_t nend = (uint32_t)__popc(emask);
         // carried from the lanes before: their last run start, the number of
         // run ends and the partition of the last one
         int start = (int)wave_excl_max((uint32_t)(last_start + 1)) - 1;
@@ -787,18 +790,13 @@ k_hs_tile_wave(const E *__restrict__ off, const int32_t *__restrict__ nbr, const
         const int64_t ob = t * (int64_t)kHubTile, idx = tile_idx_off[t];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            if (key[k] == ~0ull) continue;
-            const unsigned long long g = key[k] >> 11;
-            const unsigned long long gp = (k ? key[k - 1] : before) >> 11;
-            const unsigned long long gn = (k + 1 < EPT ? key[k + 1] : after) >> 11;
-            const int e = lane * EPT + k;
-            if ((lane == 0 && k == 0) || g != gp) start = e;
-            if ((lane == kWave - 1 && k == EPT - 1) || g != gn) {
+            if ((smask >> k) & 1u) start = lane * EPT + k;
+            if ((emask >> k) & 1u) {
                 const int pt = (int)(key[k] >> 43);
                 for (int p = pp + 1; p <= pt; ++p) tile_idx[idx + p] = (int32_t)q;   // partitions starting here
                 pp = pt;
-                out_key[ob + q] = (uint32_t)(g & 0xffffffffull);
-                out_ent[ob + q] = hub_entry((uint32_t)(e - start + 1), (uint32_t)(key[k] & (kHubTile - 1)));
+                out_key[ob + q] = (uint32_t)((key[k] >> 11) & 0xffffffffull);
+                out_ent[ob + q] = hub_entry((uint32_t)(lane * EPT + k - start + 1), (uint32_t)(key[k] & (kHubTile - 1)));
                 ++q;
             }
         }
@@ -896,7 +894,7 @@ __device__ int hs_merge_round(uint32_t *skey, uint16_t *spos, uint32_t *whist, u
 // before its start (an exclusive max-scan carries the latter, prefix sums
 // being non-decreasing), its last position that of its last entry (the
 // stable order puts the latest tile last).
-constexpr int kHsSmall = 512;
+constexpr int kHsSmall = 1024;
 constexpr int kHsSmallT = 256;
 
 struct HsWaveLds {
@@ -1025,7 +1023,8 @@ k_hs_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict
         if (n <= kWave) best = hs_wave_reduce<1>(w, n);
         else if (n <= 2 * kWave) best = hs_wave_reduce<2>(w, n);
         else if (n <= 4 * kWave) best = hs_wave_reduce<4>(w, n);
-        else best = hs_wave_reduce<8>(w, n);
+        else if (n <= 8 * kWave) best = hs_wave_reduce<8>(w, n);
+        else best = hs_wave_reduce<16>(w, n);
         if (lane == 0 && best) atomicMax(&hub_best[h], best);
         __builtin_amdgcn_wave_barrier();   // the slice is rewritten by the next item
     }
@@ -1264,7 +1263,7 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
                                      g->d_hub_nparts, g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent,
                                      g->d_tile_idx, g->d_stop);
     const SortState *S = g->sort;
-    k_hs_merge_small<<<grid_for(g, (ni + kHsSmallT / kWave - 1) / (kHsSmallT / kWave), 6), kHsSmallT, 0,
+    k_hs_merge_small<<<grid_for(g, (ni + kHsSmallT / kWave - 1) / (kHsSmallT / kWave), 3), kHsSmallT, 0,
                        g->stream>>>(g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles,
                                     g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best,
                                     S->d_big, S->d_nbig, g->d_stop);
