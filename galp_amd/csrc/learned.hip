@@ -287,12 +287,16 @@ struct CandPolicy {
     // distinct label present in the chunk (ballot of the equal lanes gives
     // its count and last position), looked up among the candidates with one
     // more ballot.  A converged row costs one round per chunk.
-    __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT]) {
+    // hid: per element the neighbour id when it is hot (< hot, so 16 bits),
+    // else 0xffff; two elements per word (element k in half k & 1 of word
+    // k / 2), which keeps the kernel within 128 VGPRs at 1024 threads.
+    static constexpr int HW = (EPT + 1) / 2;
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT], const uint32_t (&hid)[HW]) {
         uint32_t none[EPT];
-        process(it, lab, ids, none);
+        process(it, lab, hid, none);
     }
     // rpos: the row positions of the item's elements (POS only)
-    __device__ void process(const Item &it, const int32_t (&lab)[EPT], const int32_t (&ids)[EPT],
+    __device__ void process(const Item &it, const int32_t (&lab)[EPT], const uint32_t (&hid)[HW],
                             const uint32_t (&rpos)[EPT]) {
         if (it.len == 0) return;
         const int lane = lane_id();
@@ -304,8 +308,8 @@ struct CandPolicy {
         for (int k = 0; k < EPT; ++k) {
             if (k * kWave >= it.len || overflow) break;   // uniform
             const int pos = k * kWave + lane;
-            const int32_t u = ids[k];
-            const int32_t L = pos < it.len ? (u < hot ? cache[u] : lab[k]) : -1;
+            const uint32_t h = (hid[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+            const int32_t L = pos < it.len ? (h != 0xffffu ? cache[h] : lab[k]) : -1;
             const uint32_t rowpos = it.row0 + (uint32_t)(k * kWave);
             unsigned long long ball = __builtin_amdgcn_ballot_w64(L >= 0);
             while (ball) {
@@ -385,17 +389,21 @@ __device__ __forceinline__ void cand_pipeline(Policy &pol, const int32_t *__rest
             u[k] = __builtin_nontemporal_load(&nbr[ok ? e0 + pos : 0]);
         }
     };
-    auto gather = [&](int32_t (&lab)[EPT], int32_t (&ids)[EPT], const int32_t (&u)[EPT], const Item &it) {
+    auto gather = [&](int32_t (&lab)[EPT], uint32_t (&ids)[(EPT + 1) / 2], const int32_t (&u)[EPT],
+                      const Item &it) {
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * kWave + lane;
             const bool ok = it.valid && pos < it.len;
-            ids[k] = ok ? u[k] : 0;
+            const uint32_t h = ok && u[k] < hot ? (uint32_t)u[k] : 0xffffu;
+            if (k & 1) ids[k >> 1] |= h << 16;
+            else ids[k >> 1] = h;
             lab[k] = prev[(ok && u[k] >= hot) ? u[k] : 0];
         }
     };
     Item s0 = pol.item(i), s1 = pol.item(i + stride), s2 = pol.item(i + 2 * stride);
-    int32_t u[EPT], labA[EPT], labB[EPT], idA[EPT], idB[EPT];
+    int32_t u[EPT], labA[EPT], labB[EPT];
+    uint32_t idA[(EPT + 1) / 2], idB[(EPT + 1) / 2];   // hot ids, two per word
     load_nbr(u, s0);
     gather(labA, idA, u, s0);
     load_nbr(u, s1);
@@ -433,20 +441,23 @@ __device__ __forceinline__ void cand_pipeline_pos(Policy &pol, const int32_t *__
             u[k] = __builtin_nontemporal_load(&nbr[ok ? e0 + pos : 0]);
         }
     };
-    auto gather = [&](int32_t (&lab)[EPT], int32_t (&ids)[EPT], uint32_t (&ps)[EPT], const int32_t (&u)[EPT],
+    auto gather = [&](int32_t (&lab)[EPT], uint32_t (&ids)[(EPT + 1) / 2], uint32_t (&ps)[EPT], const int32_t (&u)[EPT],
                       const Item &it) {
         const int64_t e0 = it.valid ? it.e0 : 0;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int pos = k * kWave + lane;
             const bool ok = it.valid && pos < it.len;
-            ids[k] = ok ? u[k] : 0;
+            const uint32_t h = ok && u[k] < hot ? (uint32_t)u[k] : 0xffffu;
+            if (k & 1) ids[k >> 1] |= h << 16;
+            else ids[k >> 1] = h;
             lab[k] = prev[(ok && u[k] >= hot) ? u[k] : 0];
             ps[k] = __builtin_nontemporal_load(&pos_src[ok ? e0 + pos : 0]);
         }
     };
     Item s0 = pol.item(i), s1 = pol.item(i + stride), s2 = pol.item(i + 2 * stride);
-    int32_t u[EPT], labA[EPT], labB[EPT], idA[EPT], idB[EPT];
+    int32_t u[EPT], labA[EPT], labB[EPT];
+    uint32_t idA[(EPT + 1) / 2], idB[(EPT + 1) / 2];   // hot ids, two per word
     uint32_t pA[EPT], pB[EPT];
     load_nbr(u, s0);
     gather(labA, idA, pA, u, s0);
@@ -480,12 +491,13 @@ struct CandLists {
     const uint32_t *fpos;
 };
 
-// One block per CU: the LDS cache takes 152 KB of the CU's 160 KB.  768
-// threads (12 waves) leave 168 VGPRs per lane: the kernel needs 142 and
-// spills nothing.  At 1024 threads it was held to 128 VGPRs and spilled 41
-// to scratch, and ran no faster (RMAT-26: 9.39 against 9.40 ms per launch);
-// 512 threads (8 waves) ran 16 % slower on RMAT-22.
-constexpr int kCandThreads = 768;
+// One block per CU: the LDS cache takes 152 KB of the CU's 160 KB.  1024
+// threads (16 waves) hold the kernel to 128 VGPRs, which it fits with the
+// hot ids packed two to a register.  Unpacked it needed 142 and spilled 41
+// to scratch at 1024 threads; 768 threads (12 waves, no spill) ran as fast
+// at RMAT-26 but 17 % slower at RMAT-22 (418 against 357 µs per launch),
+// and 512 threads (8 waves) 16 % slower there.
+constexpr int kCandThreads = 1024;
 
 template <typename E>
 __global__ void __launch_bounds__(kCandThreads)
