@@ -297,10 +297,10 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     for (size_t h = 0; h < hub_vertices.size(); ++h) {
         const int32_t x = hub_vertices[h];
         const int64_t d = offi[(size_t)x + 1] - offi[(size_t)x];
-        if (d > (int64_t)kHubPart * kHubMaxParts)
-            return fail(ctx, GALP_EUNSUPPORTED, "a vertex of degree " + std::to_string(d) +
-                                                    " exceeds the hub path's limit of " +
-                                                    std::to_string((int64_t)kHubPart * kHubMaxParts));
+        // partitions are capped at kHubMaxParts (the tile kernels' offsets):
+        // beyond kHubPart * kHubMaxParts edges a partition may hold more
+        // distinct labels than a merge table, and the merges then take it in
+        // several rounds
         const int64_t part = g->opts.method == GALP_METHOD_SORT ? kHubPartSort : kHubPart;
         const int32_t P = (int32_t)std::min<int64_t>(kHubMaxParts, (d + part - 1) / part);
         const int32_t nt = (int32_t)((d + kHubTile - 1) / kHubTile);

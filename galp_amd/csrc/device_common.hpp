@@ -27,6 +27,13 @@ __device__ __forceinline__ uint32_t hub_partition(uint32_t L, uint32_t P) {
     return (uint32_t)(((unsigned long long)(L * 0x85EBCA77u) * P) >> 32);
 }
 
+// Round of a label when one hub partition is merged in R rounds (R a power
+// of two, at most 4096): bits of a third multiplicative hash, so labels
+// that share a partition spread over the rounds.
+__device__ __forceinline__ uint32_t sub_round(uint32_t L, uint32_t R) {
+    return ((L * 0xC2B2AE3Du) >> 20) & (R - 1u);
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
 // ---------------------------------------------------------------------------

@@ -52,8 +52,8 @@ __host__ __device__ constexpr uint32_t hub_entry(uint32_t count, uint32_t tile_p
 __host__ __device__ constexpr uint32_t hub_entry_count(uint32_t w) { return w >> kHubTileBits; }
 __host__ __device__ constexpr uint32_t hub_entry_pos(uint32_t w) { return w & (kHubTile - 1); }
 constexpr int kHubTileCapBits = 12;   // LDS table of a tile (4096 slots)
-constexpr int kHubPart = 2048;        // hub partitions: ceil(deg / kHubPart)
-constexpr int kHubMaxParts = 4096;    // so hub degree <= kHubPart * kHubMaxParts
+constexpr int kHubPart = 2048;        // hub partitions: min(kHubMaxParts, ceil(deg / kHubPart))
+constexpr int kHubMaxParts = 4096;
 // The sort method cuts hubs finer, min(kHubMaxParts, ceil(deg / 512)): a
 // partition then holds about 512 entries even when every label of the row
 // is distinct (the first iterations), which one wave merges in registers.

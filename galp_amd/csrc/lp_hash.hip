@@ -135,6 +135,30 @@ __device__ __forceinline__ uint32_t lds_insert(uint32_t *skey, uint32_t L, bool 
     return h;
 }
 
+// lds_insert for callers that recover from a full table: returns kTableFull
+// (no slot taken) instead of flagging an error.
+constexpr uint32_t kTableFull = 0xffffffffu;
+template <int BITS>
+__device__ __forceinline__ uint32_t lds_try_insert(uint32_t *skey, uint32_t L, bool *created) {
+    constexpr uint32_t MASK = (1u << BITS) - 1u;
+    uint32_t h = hash_slot(L, BITS);
+    *created = false;
+    for (uint32_t probe = 0; probe <= MASK; ++probe) {
+        const uint32_t cur = skey[h];
+        if (cur == L) return h;
+        if (cur == kEmpty) {
+            const uint32_t old = atomicCAS(&skey[h], kEmpty, L);
+            if (old == kEmpty) {
+                *created = true;
+                return h;
+            }
+            if (old == L) return h;
+        }
+        h = (h + 1) & MASK;
+    }
+    return kTableFull;
+}
+
 // Inserts one 64-lane chunk of labels (L < 0: no edge) into an LDS table,
 // adding each label's count (and, if LAST, the max of its positions `pos`).
 // Up to PEEL labels are committed by a single leader lane after a ballot --
@@ -504,6 +528,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     __shared__ int32_t sscan_w[T / kWave];
     __shared__ int32_t stotal;
     __shared__ unsigned long long sred64[T / kWave];
+    __shared__ int32_t sfull;   // a label found the table full in this round
     if (*stop || (nsel && *nsel == 0)) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
@@ -511,11 +536,20 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         slast[s] = 0;
     }
     if (threadIdx.x == 0) nlist = 0;
+    if (threadIdx.x == 0) sfull = 0;
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int h = item_hub[it];
         if (hub_sel && !hub_sel[h]) continue;   // uniform per block
         const int p = item_part[it];
         const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        // A partition with more distinct labels than the table holds (an
+        // adversarial label set, or a hub whose partitions were capped at
+        // kHubMaxParts) is merged again in R rounds, round r taking the
+        // labels whose sub-round hash is r; R doubles until every round fits.
+        uint32_t R = 1;
+        unsigned long long item_best = 0;
+        uint32_t item_nl = 0;
+        for (uint32_t r = 0; r < R;) {
         // tiles in chunks of T: the chunk's slice bounds are loaded one tile
         // per thread (no chain of dependent index loads), then the block
         // strides over the chunk's slices concatenated (an exclusive scan of
@@ -549,10 +583,17 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
                         else b = mid;
                     }
                     const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - shi[a]);
-                    const uint32_t w = out_ent[e];
-                    sl = lds_insert<kHubMergeCapBits>(skey, out_key[e], &created, err);
-                    atomicAdd(&scnt[sl], hub_entry_count(w));
-                    atomicMax(&slast[sl], (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(w));
+                    const uint32_t key = out_key[e];
+                    if (R == 1 || sub_round(key, R) == r) {
+                        const uint32_t w = out_ent[e];
+                        sl = lds_try_insert<kHubMergeCapBits>(skey, key, &created);
+                        if (sl == kTableFull) {
+                            sfull = 1;
+                        } else {
+                            atomicAdd(&scnt[sl], hub_entry_count(w));
+                            atomicMax(&slast[sl], (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(w));
+                        }
+                    }
                 }
                 // one list append per wave
                 const unsigned long long ball = LIST ? __ballot(created) : 0ull;
@@ -592,13 +633,32 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
                 }
         }
         __syncthreads();
-        if (threadIdx.x == 0) nlist = 0;
-        best = block_max<T / kWave>(best, sred64);
+        const bool full = sfull != 0;   // uniform: read after the barrier
+        best = block_max<T / kWave>(best, sred64);   // barriers inside: every read above is done
         if constexpr (!LIST) {
             if (hub_distinct) nl = block_sum<T / kWave>(nl, (uint32_t *)sscan_w);   // uniform branch
         }
-        if (threadIdx.x == 0 && best) atomicMax(&hub_best[h], best);
-        if (threadIdx.x == 0 && hub_distinct) atomicAdd(&hub_distinct[h], nl);
+        if (threadIdx.x == 0) {
+            nlist = 0;
+            sfull = 0;
+        }
+        if (full) {   // start the item over in twice as many rounds
+            if (R >= (1u << kHubMergeCapBits)) {
+                if (threadIdx.x == 0) atomicOr(err, 1);
+                break;
+            }
+            R *= 2;
+            r = 0;
+            item_best = 0;
+            item_nl = 0;
+            continue;
+        }
+        item_best = best > item_best ? best : item_best;
+        item_nl += nl;
+        ++r;
+        }
+        if (threadIdx.x == 0 && item_best) atomicMax(&hub_best[h], item_best);
+        if (threadIdx.x == 0 && hub_distinct) atomicAdd(&hub_distinct[h], item_nl);
     }
 }
 

@@ -671,6 +671,7 @@ k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 //  3. final (k_hs_final): the winner is the label at position `last`.
 constexpr int kHsMergeT = 1024;
 constexpr int kHsMergeCap = 4096;                 // entries staged per round
+constexpr uint32_t kMergePad = 0xffffffffu;       // staged in place of another round's labels
 
 // Wave-wide exclusive scans (sum, max) of one value per lane.
 __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t x) {
@@ -855,7 +856,8 @@ __device__ int hs_merge_round(uint32_t *skey, uint16_t *spos, uint32_t *whist, u
     for (int k = 0; k < EPT; ++k) {
         const int q = tid * EPT + k;
         end[k] = false;
-        if (q < len && (q == len - 1 || kf[q + 1] != kf[q])) {
+        // a run of pads (labels of another round) is dropped: it sorts last
+        if (q < len && kf[q] != kMergePad && (q == len - 1 || kf[q + 1] != kf[q])) {
             const uint32_t key = kf[q];
             int lo = 0, hi = q;
             while (lo < hi) {
@@ -1057,6 +1059,12 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
         const int h = item_hub[it];
         const int p = item_part[it];
         const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        // More distinct labels than N in the partition: merged again in R
+        // rounds, round r staging the labels whose sub-round hash is r (the
+        // others as pads); R doubles until every round fits.
+        uint32_t R = 1;
+        unsigned long long item_best = 0;
+        for (uint32_t r = 0; r < R;) {
         int held = 0;   // staged slots: reduced runs first, then new entries (uniform)
         bool overflow = false;
         for (int64_t c0 = t0; c0 < t0 + nt && !overflow; c0 += T) {
@@ -1080,7 +1088,7 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
             for (int i0 = 0; i0 < total;) {
                 if (held == N) {   // full: merge what is staged
                     held = hs_merge_round<T, N>(skey, spos, whist, scnt, slast, held, passes, sscan, &snruns);
-                    if (held == N) {   // N distinct labels in one partition: no room to progress
+                    if (held == N) {   // N distinct labels in this round: no room to progress
                         overflow = true;
                         break;
                     }
@@ -1095,9 +1103,11 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
                         else b = mid;
                     }
                     const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - sstart[a]);
+                    const uint32_t key = out_key[e];
+                    const bool mine = R == 1 || sub_round(key, R) == r;
                     const uint32_t w = out_ent[e];
-                    skey[held + j] = out_key[e];
-                    scnt[held + j] = hub_entry_count(w);
+                    skey[held + j] = mine ? key : kMergePad;
+                    scnt[held + j] = mine ? hub_entry_count(w) : 0u;
                     slast[held + j] = (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(w);
                 }
                 held += take;
@@ -1105,18 +1115,27 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
                 __syncthreads();
             }
         }
-        if (overflow) {
-            if (tid == 0) atomicOr(err, 16);
+        if (overflow) {   // start the item over in twice as many rounds
+            if (R >= 4096u) {
+                if (tid == 0) atomicOr(err, 16);
+                break;
+            }
+            R *= 2;
+            r = 0;
+            item_best = 0;
             continue;
         }
         if (held > 0) held = hs_merge_round<T, N>(skey, spos, whist, scnt, slast, held, passes, sscan, &snruns);
         unsigned long long best = 0;
-        for (int r = tid; r < held; r += T) {
-            const unsigned long long pk = ((unsigned long long)scnt[r] << 32) | slast[r];
+        for (int q = tid; q < held; q += T) {
+            const unsigned long long pk = ((unsigned long long)scnt[q] << 32) | slast[q];
             best = pk > best ? pk : best;
         }
         best = block_max<T / kWave>(best, sred64);
-        if (tid == 0 && best) atomicMax(&hub_best[h], best);
+        item_best = best > item_best ? best : item_best;
+        ++r;
+        }
+        if (tid == 0 && item_best) atomicMax(&hub_best[h], item_best);
     }
 }
 

@@ -153,6 +153,59 @@ def test_hub_from_clustered_labels(ctx, method):
         assert st["updated"] == cnt
 
 
+def partition0_labels(n, parts, count, rng):
+    """`count` distinct vertex ids whose hub partition is 0 of `parts` (the
+    multiplicative hash the hub path partitions labels by), so that one
+    partition of a hub receives all of them."""
+    ids = np.arange(n, dtype=np.uint64)
+    h = (ids * np.uint64(0x85EBCA77)) & np.uint64(0xFFFFFFFF)
+    ok = np.nonzero(((h * np.uint64(parts)) >> np.uint64(32)) == 0)[0]
+    return rng.choice(ok, size=count, replace=False).astype(np.int32)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_hub_partition_overflow(ctx, method):
+    # A hub of 6000 edges whose neighbours carry 5000 distinct labels, all in
+    # one label partition (of 3 for the hash method, of 12 for the sort
+    # method), more than a merge table holds: the merges must take that
+    # partition in rounds and still give the oracle's labels.
+    n, deg = 100_000, 6000
+    rng = np.random.default_rng(11)
+    rows = [[] for _ in range(n)]
+    for u in range(1, deg + 1):
+        rows[0].append(u)
+        rows[u].append(0)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(r) for r in rows])
+    nbr = np.array([u for r in rows for u in r], np.int32)
+    pool = partition0_labels(n, 12, 5000, rng)
+    prev = np.arange(n, dtype=np.int32)
+    prev[1:5001] = pool
+    prev[5001:deg + 1] = rng.choice(pool, size=deg - 5000)   # repeats: counts of 1 and 2
+    ref, cnt = oracle.lp_iterate(off, nbr, prev)
+    with _abi.Graph(ctx, off, nbr, method=method, reorder=0) as g:
+        g.set_labels(prev)
+        st = g.step()
+        assert np.array_equal(g.labels(), ref)
+        assert st["updated"] == cnt
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_hub_beyond_partition_cap(ctx, method):
+    # A star of 17.5 M leaves: the hub's partitions are capped at 4096, so in
+    # the first iteration (every label distinct) each holds ~4270 labels,
+    # more than a merge table: merged in rounds.  Once refused at upload.
+    leaves = 17_500_000
+    n = leaves + 1
+    off = np.empty(n + 1, np.int64)
+    off[0], off[1] = 0, leaves
+    off[2:] = leaves + np.arange(1, leaves + 1, dtype=np.int64)
+    nbr = np.empty(2 * leaves, np.int32)
+    nbr[:leaves] = np.arange(1, n, dtype=np.int32)
+    nbr[leaves:] = 0
+    check_per_iteration(ctx, off, nbr, 2, f"star{leaves}", method=method)
+
+
 @pytest.mark.parametrize("method", METHODS)
 def test_device_loop_equals_steps(ctx, method):
     off, nbr = _abi.gen_rmat(14, 16, seed=3)
