@@ -128,7 +128,8 @@ static void dev_free_all(galp_graph *g) {
     g->gexec = nullptr;
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
                     g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
-                    g->d_hub_best, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub, g->d_tile_idx_off,
+                    g->d_hub_best, g->d_merge_big, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub,
+                    g->d_tile_idx_off,
                     g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts, g->d_err};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -333,8 +334,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         int64_t need = resident + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
         // the hub plan of both methods: merge items, tile outputs and partition
         // offsets; the sort method's list of large merge items
-        need += 8 * g->nhub_items + (8 + 8 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
-        if (g->opts.method == GALP_METHOD_SORT) need += 8 * g->nhub_items + 8;
+        need += 12 * g->nhub_items + (8 + 8 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
         if (g->reorder) need += 8 * n + (ooc ? 0 : 4 * m + 8 * (n + 1));  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;
@@ -405,6 +405,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_hub_vid, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_nparts, g->nhubs));
         GALP_TRY(dev_alloc(g, &g->d_hub_best, g->nhubs));
+        GALP_TRY(dev_alloc(g, &g->d_merge_big, g->nhub_items));
         auto up = [&](void *dst, const void *src, size_t bytes) {
             return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
         };
@@ -428,6 +429,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
         GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
+        GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_merge_big, g->nhub_items, 0u));
     }
     if (ooc) {
         std::vector<int64_t> hub_item0(hub_nparts.size() + 1, 0);

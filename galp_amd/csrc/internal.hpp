@@ -189,9 +189,6 @@ struct LearnState {
 // labels < n.  Its hub path uses the hub plan both methods share.
 struct SortState {
     int passes = 0;
-    // hub merge items with more entries than one wave merges: their list
-    // and count (reset by the hub final kernel)
-    int64_t *d_big = nullptr, *d_nbig = nullptr;
 };
 
 
@@ -282,6 +279,12 @@ struct galp_graph {
     int32_t *d_item_hub = nullptr;          // hub of each merge item
     int32_t *d_item_part = nullptr;         // partition of each merge item
     unsigned long long *d_hub_best = nullptr;  // max (count << 32 | last) per hub
+    // per merge item, set by the wave merge in every iteration: 1 when the
+    // item has more entries (or distinct labels) than one wave merges, and
+    // the block merge takes it.  A flag per item rather than a list: one
+    // append per item serialised ~400,000 same-address atomics per
+    // iteration at RMAT-26 when every item was large.
+    int32_t *d_merge_big = nullptr;
 
     // iteration control (device)
     unsigned long long *d_counter = nullptr;

@@ -508,11 +508,135 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
     pipelined<T, EPT>(pol, nbr, prev, tbase + (int64_t)blockIdx.x, (int64_t)gridDim.x, (int)threadIdx.x);
 }
 
+// Merge items with at most kHmSmall entries -- the usual case once labels
+// have settled, when a partition holds about one entry per tile -- by one
+// wave each: the wave walks the item's entries 64 at a time and peels each
+// batch label by label (one ballot per distinct label, a wave sum of its
+// counts and a wave max of its last positions), keeping up to 64 distinct
+// labels in lanes (lane j: label, count, last position of candidate j).
+// Items with more entries, or more than 64 distinct labels, are flagged for
+// the block merge (k_hub_merge); waves need no barrier.
+constexpr int kHmSmall = 1024;
+constexpr int kHmSmallT = 256;
+
+__global__ void __launch_bounds__(kHmSmallT)
+k_hub_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
+                  int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
+                  const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
+                  const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
+                  unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ hub_sel,
+                  uint32_t *__restrict__ hub_distinct, const int64_t *__restrict__ nsel, int32_t *__restrict__ big,
+                  const int32_t *__restrict__ stop) {
+    constexpr int NW = kHmSmallT / kWave;
+    __shared__ int32_t slo[NW][kWave], sstart[NW][kWave];
+    if (*stop || (nsel && *nsel == 0)) return;
+    const int lane = lane_id(), w = (int)(threadIdx.x / kWave);
+    const int64_t wid = (int64_t)blockIdx.x * NW + w;
+    for (int64_t it = ibase + wid; it < ibase + nitems; it += (int64_t)gridDim.x * NW) {
+        const int h = item_hub[it];
+        if (hub_sel && !hub_sel[h]) continue;   // uniform per wave
+        const int p = item_part[it];
+        const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        int total = 0;
+        for (int64_t c0 = t0; c0 < t0 + nt; c0 += kWave) {
+            const int64_t t = c0 + lane;
+            int len = 0;
+            if (t < t0 + nt) {
+                const int64_t ib = tile_idx_off[t];
+                len = tile_idx[ib + p + 1] - tile_idx[ib + p];
+            }
+            total += (int)wave_sum_u32((uint32_t)len);
+        }
+        if (total == 0) {
+            if (lane == 0) big[it] = 0;
+            continue;
+        }
+        bool over = total > kHmSmall;   // uniform
+        int32_t ckv = -1;   // candidate label of this lane (-1: none)
+        uint32_t ccv = 0, clv = 0;
+        int ncand = 0;      // uniform
+        for (int64_t c0 = t0; c0 < t0 + nt && !over; c0 += kWave) {
+            const int64_t t = c0 + lane;
+            int lo = 0, len = 0;
+            if (t < t0 + nt) {
+                const int64_t ib = tile_idx_off[t];
+                lo = tile_idx[ib + p];
+                len = tile_idx[ib + p + 1] - lo;
+            }
+            int st = len;   // exclusive scan of the slice lengths
+#pragma unroll
+            for (int sft = 1; sft < kWave; sft <<= 1) {
+                const int y = __shfl_up(st, sft);
+                if (lane >= sft) st += y;
+            }
+            const int ctot = __shfl(st, kWave - 1);
+            st -= len;
+            slo[w][lane] = lo;
+            sstart[w][lane] = st;
+            __builtin_amdgcn_wave_barrier();
+            const int nc = (int)min((int64_t)kWave, t0 + nt - c0);
+            for (int i0 = 0; i0 < ctot && !over; i0 += kWave) {
+                const int i = i0 + lane;
+                int32_t L = -1;
+                uint32_t c = 0, last = 0;
+                if (i < ctot) {
+                    int a = 0, b = nc;   // last tile with start <= i
+                    while (b - a > 1) {
+                        const int mid = (a + b) >> 1;
+                        if (sstart[w][mid] <= i) a = mid;
+                        else b = mid;
+                    }
+                    const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[w][a] + (i - sstart[w][a]);
+                    const uint32_t ent = out_ent[e];
+                    L = (int32_t)out_key[e];
+                    c = hub_entry_count(ent);
+                    last = (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(ent);
+                }
+                unsigned long long ball = __ballot(L >= 0);
+                while (ball) {
+                    const int32_t Lk = __builtin_amdgcn_readlane(L, __ffsll((long long)ball) - 1);
+                    const bool mine = L == Lk;
+                    ball &= ~__ballot(mine);
+                    const uint32_t cs = wave_sum_u32(mine ? c : 0u);
+                    const uint32_t lm = wave_max(mine ? last : 0u);
+                    const unsigned long long hit = __ballot(ckv == Lk);
+                    if (hit) {
+                        if (lane == __ffsll((long long)hit) - 1) {
+                            ccv += cs;
+                            clv = lm > clv ? lm : clv;
+                        }
+                    } else {
+                        if (ncand == kWave) {
+                            over = true;
+                            break;
+                        }
+                        if (lane == ncand) {
+                            ckv = Lk;
+                            ccv = cs;
+                            clv = lm;
+                        }
+                        ++ncand;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();   // the slice bounds are rewritten by the next chunk
+        }
+        if (lane == 0) big[it] = over ? 1 : 0;
+        if (over) continue;   // the block merge takes it
+        const unsigned long long best = wave_max(lane < ncand ? (((unsigned long long)ccv << 32) | clv) : 0ull);
+        if (lane == 0) {
+            if (best) atomicMax(&hub_best[h], best);
+            if (hub_distinct) atomicAdd(&hub_distinct[h], (uint32_t)ncand);
+        }
+    }
+}
+
 // LIST: keep a list of the slots created (argmax and reset visit only them);
 // without it they scan the whole table, which frees 16 KB of LDS per block.
 template <int T, bool LIST>
 __global__ void __launch_bounds__(T)
-k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t nitems,
+k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
+            int64_t nitems, const int32_t *__restrict__ big,
             const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
             const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
             const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
@@ -529,6 +653,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     __shared__ int32_t stotal;
     __shared__ unsigned long long sred64[T / kWave];
     __shared__ int32_t sfull;   // a label found the table full in this round
+    __shared__ uint32_t sbig[T / 32];   // flags of the current chunk of items
     if (*stop || (nsel && *nsel == 0)) return;
     for (int s = threadIdx.x; s < CAP; s += T) {
         skey[s] = kEmpty;
@@ -537,7 +662,29 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     }
     if (threadIdx.x == 0) nlist = 0;
     if (threadIdx.x == 0) sfull = 0;
-    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    // the flagged items of each chunk of T of the block's items, gathered
+    // into an LDS bit mask (one flag load per thread instead of a dependent
+    // load per item)
+    // items go round-robin over the blocks (item ibase + blockIdx.x + j *
+    // gridDim.x is the block's j-th), T of them per chunk
+    const int64_t iend = ibase + nitems;
+    for (int64_t j0 = 0; ibase + blockIdx.x + j0 * gridDim.x < iend; j0 += T) {
+    __syncthreads();   // the previous chunk's reads of sbig are done
+    if (threadIdx.x < T / 32) sbig[threadIdx.x] = 0u;
+    __syncthreads();
+    {
+        const int64_t ij = ibase + blockIdx.x + (j0 + threadIdx.x) * gridDim.x;
+        if (ij < iend && big[ij]) atomicOr(&sbig[threadIdx.x / 32], 1u << (threadIdx.x % 32));
+    }
+    __syncthreads();
+    for (int q = 0; q < T; ++q) {
+        const uint32_t word = sbig[q / 32] >> (q % 32);   // uniform
+        if (word == 0u) {   // no flagged item left in this word
+            q |= 31;
+            continue;
+        }
+        q += __ffs((int)word) - 1;
+        const int64_t it = ibase + blockIdx.x + (j0 + q) * gridDim.x;
         const int h = item_hub[it];
         if (hub_sel && !hub_sel[h]) continue;   // uniform per block
         const int p = item_part[it];
@@ -659,6 +806,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         }
         if (threadIdx.x == 0 && item_best) atomicMax(&hub_best[h], item_best);
         if (threadIdx.x == 0 && hub_distinct) atomicAdd(&hub_distinct[h], item_nl);
+    }
     }
 }
 
@@ -796,10 +944,14 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
             g->d_err, sel, nsel, g->learn->d_hub_hd);
     prof_end(g, &pl);
     prof_begin(g, "hub_merge", 0.0, &pl);
+    k_hub_merge_small<<<grid_for(g, (ni + kHmSmallT / kWave - 1) / (kHmSmallT / kWave), 8), kHmSmallT, 0,
+                        g->stream>>>(g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles,
+                                     g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best,
+                                     sel, hdist, nsel, g->d_merge_big, g->d_stop);
     k_hub_merge<TM, LIST><<<grid_for(g, ni, TM == 256 || !LIST ? 3 : 2), TM, 0, g->stream>>>(
-        g->d_item_hub + w.i0, g->d_item_part + w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
-        g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, sel, hdist,
-        nsel);
+        g->d_item_hub, g->d_item_part, w.i0, ni, g->d_merge_big, g->d_hub_tile0, g->d_hub_ntiles,
+        g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, sel,
+        hdist, nsel);
     prof_end(g, &pl);
     prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);
     k_hub_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(

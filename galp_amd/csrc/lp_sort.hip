@@ -658,7 +658,7 @@ k_sort_lds(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 //     writes them with the P_h + 1 partition offsets, the same layout as
 //     the hash method's tiles;
 //  2. merge: items with at most kHsSmall entries are merged by one wave
-//     each (k_hs_merge_small), the others are listed for k_hs_merge: a block
+//     each (k_hs_merge_small), the others are flagged for k_hs_merge: a block
 //     per (hub, partition) stages that
 //     partition's entries of every tile in LDS, tiles in row order, sorts
 //     them by label (stable radix sort, entry index as payload) and reduces
@@ -888,7 +888,7 @@ __device__ int hs_merge_round(uint32_t *skey, uint16_t *spos, uint32_t *whist, u
 
 // Small merge items: one wave per item (hub, partition) whose tiles hold at
 // most kHsSmall entries in it -- the usual case once labels have settled,
-// when a partition gets about one entry per tile.  Larger items go to a list
+// when a partition gets about one entry per tile.  Larger items are flagged
 // for the block merge (k_hs_merge).  The wave stages the item's entries in
 // its LDS slice (flat index = tile order), sorts (label << 32 | flat index)
 // with the register bitonic network, and reduces each run: its count is the
@@ -963,7 +963,7 @@ k_hs_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict
                  int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
                  const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
                  const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
-                 unsigned long long *__restrict__ hub_best, int64_t *__restrict__ big, int64_t *__restrict__ nbig,
+                 unsigned long long *__restrict__ hub_best, int32_t *__restrict__ big,
                  const int32_t *__restrict__ stop) {
     constexpr int NW = kHsSmallT / kWave;
     __shared__ HsWaveLds sw[NW];
@@ -985,11 +985,8 @@ k_hs_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict
             }
             total += (int)wave_sum_u32((uint32_t)len);
         }
-        if (total == 0) continue;
-        if (total > kHsSmall) {   // the block merge takes it
-            if (lane == 0) big[atomicAdd((unsigned long long *)nbig, 1ull)] = it;
-            continue;
-        }
+        if (lane == 0) big[it] = total > kHsSmall ? 1 : 0;
+        if (total == 0 || total > kHsSmall) continue;   // nothing, or the block merge takes it
         int n = 0;
         for (int64_t c0 = t0; c0 < t0 + nt; c0 += kWave) {
             const int64_t t = c0 + lane;
@@ -1036,7 +1033,7 @@ k_hs_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict
 template <int T>
 __global__ void __launch_bounds__(T)
 k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part,
-           const int64_t *__restrict__ big, const int64_t *__restrict__ nbig,
+           int64_t ibase, int64_t nitems, const int32_t *__restrict__ big,
            const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
            const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
            const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
@@ -1051,11 +1048,32 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
     __shared__ int sscan[T / kWave];
     __shared__ int stotal, snruns;
     __shared__ unsigned long long sred64[T / kWave];
+    __shared__ uint32_t sbig[T / 32];   // flags of the current chunk of items
     if (*stop) return;
     const int tid = (int)threadIdx.x;
-    const int64_t nitems = *nbig;
-    for (int64_t k = blockIdx.x; k < nitems; k += gridDim.x) {
-        const int64_t it = big[k];
+    // the flagged items of each chunk of T of the block's items, gathered
+    // into an LDS bit mask (one flag load per thread instead of a dependent
+    // load per item)
+    // items go round-robin over the blocks (item ibase + blockIdx.x + j *
+    // gridDim.x is the block's j-th), T of them per chunk
+    const int64_t iend = ibase + nitems;
+    for (int64_t j0 = 0; ibase + blockIdx.x + j0 * gridDim.x < iend; j0 += T) {
+    __syncthreads();   // the previous chunk's reads of sbig are done
+    if (tid < T / 32) sbig[tid] = 0u;
+    __syncthreads();
+    {
+        const int64_t ij = ibase + blockIdx.x + (j0 + tid) * gridDim.x;
+        if (ij < iend && big[ij]) atomicOr(&sbig[tid / 32], 1u << (tid % 32));
+    }
+    __syncthreads();
+    for (int qq = 0; qq < T; ++qq) {
+        const uint32_t word = sbig[qq / 32] >> (qq % 32);   // uniform
+        if (word == 0u) {   // no flagged item left in this word
+            qq |= 31;
+            continue;
+        }
+        qq += __ffs((int)word) - 1;
+        const int64_t it = ibase + blockIdx.x + (j0 + qq) * gridDim.x;
         const int h = item_hub[it];
         const int p = item_part[it];
         const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
@@ -1137,6 +1155,7 @@ k_hs_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ ite
         }
         if (tid == 0 && item_best) atomicMax(&hub_best[h], item_best);
     }
+    }
 }
 
 // Hub winners: the label at position `last` of the row; resets hub_best.
@@ -1144,10 +1163,8 @@ template <typename E>
 __global__ void __launch_bounds__(256)
 k_hs_final(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int32_t *__restrict__ hub_vid,
            int64_t nhubs, unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ prev,
-           int32_t *__restrict__ next, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop,
-           int64_t *__restrict__ nbig) {
+           int32_t *__restrict__ next, unsigned long long *__restrict__ counter, const int32_t *__restrict__ stop) {
     __shared__ uint32_t sred[4];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *nbig = 0;   // the large-item list is used up
     if (*stop) return;
     uint32_t changed = 0;
     for (int64_t base = (int64_t)blockIdx.x * 256; base < nhubs; base += (int64_t)gridDim.x * 256) {
@@ -1187,10 +1204,6 @@ static int salloc(galp_graph *g, T **p, int64_t count) {
 }
 
 void sort_free(galp_graph *g) {
-    if (g->sort) {
-        if (g->sort->d_big) (void)hipFree(g->sort->d_big);
-        if (g->sort->d_nbig) (void)hipFree(g->sort->d_nbig);
-    }
     delete g->sort;
     g->sort = nullptr;
 }
@@ -1204,10 +1217,7 @@ int sort_setup(galp_graph *g) {
     g->sort = S;
     const int bits = g->n > 1 ? 64 - __builtin_clzll((unsigned long long)(g->n - 1)) : 1;
     S->passes = (bits + 7) / 8;
-    int rc = salloc(g, &S->d_big, g->nhub_items);
-    if (rc == GALP_OK) rc = salloc(g, &S->d_nbig, 1);
-    if (rc == GALP_OK) rc = dev_fill_u32(g, (uint32_t *)S->d_nbig, 2, 0u);
-    return rc;
+    return GALP_OK;
 }
 
 template <typename E>
@@ -1285,13 +1295,12 @@ static int sort_hubs(galp_graph *g, const IterView &w, const int32_t *prev, int3
     k_hs_merge_small<<<grid_for(g, (ni + kHsSmallT / kWave - 1) / (kHsSmallT / kWave), 3), kHsSmallT, 0,
                        g->stream>>>(g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles,
                                     g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best,
-                                    S->d_big, S->d_nbig, g->d_stop);
+                                    g->d_merge_big, g->d_stop);
     k_hs_merge<kHsMergeT><<<grid_for(g, ni, merge_per_cu), kHsMergeT, 0, g->stream>>>(
-        g->d_item_hub, g->d_item_part, S->d_big, S->d_nbig, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
+        g->d_item_hub, g->d_item_part, w.i0, ni, g->d_merge_big, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_idx_off,
         g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, S->passes);
     k_hs_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
-        (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop,
-        S->d_nbig);
+        (const E *)w.off, w.nbr, g->d_hub_vid + w.h0, nh, g->d_hub_best + w.h0, prev, next, g->d_counter, g->d_stop);
     prof_end(g, &pl);
     return GALP_OK;
 }
