@@ -127,7 +127,7 @@ static void dev_free_all(galp_graph *g) {
     if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
     g->gexec = nullptr;
     void *ptrs[] = {g->d_off, g->d_nbr, g->d_lab[0], g->d_lab[1], g->d_perm, g->d_iperm, g->d_vlist,
-                    g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part,
+                    g->d_out, g->d_hub_vid, g->d_hub_nparts, g->d_item_hub, g->d_item_part, g->d_item_meta,
                     g->d_hub_best, g->d_merge_big, g->d_hub_tile0, g->d_hub_ntiles, g->d_tile_hub,
                     g->d_tile_idx_off,
                     g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_counter, g->d_stop, g->d_iters_done, g->d_iter_counts, g->d_err};
@@ -294,6 +294,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
     // hubs: phase-1 tiles and phase-2 (hub, label partition) items
     std::vector<int32_t> hub_nparts, item_hub, item_part, hub_tile0, hub_ntiles, tile_hub;
     std::vector<int64_t> tile_idx_off;
+    std::vector<HubItem> item_meta;
     int64_t ntile_idx = 0;
     for (size_t h = 0; h < hub_vertices.size(); ++h) {
         const int32_t x = hub_vertices[h];
@@ -308,6 +309,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         hub_nparts.push_back(P);
         hub_tile0.push_back((int32_t)tile_hub.size());
         hub_ntiles.push_back(nt);
+        const int64_t idx0 = ntile_idx;
         for (int32_t t = 0; t < nt; ++t) {
             tile_hub.push_back((int32_t)h);
             tile_idx_off.push_back(ntile_idx);
@@ -316,6 +318,7 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         for (int32_t p = 0; p < P; ++p) {
             item_hub.push_back((int32_t)h);
             item_part.push_back(p);
+            item_meta.push_back(HubItem{(int32_t)h, p, hub_tile0.back(), nt, P, 0, idx0});
         }
     }
     g->nhubs = (int64_t)hub_vertices.size();
@@ -334,7 +337,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         int64_t need = resident + esz * (n + 1) + 4 * 3 * n + 32 * g->nhubs + 4 * g->nhub_tiles + 4096;
         // the hub plan of both methods: merge items, tile outputs and partition
         // offsets; the sort method's list of large merge items
-        need += 12 * g->nhub_items + (8 + 8 * kHubTile) * g->nhub_tiles + 4 * ntile_idx;
+        need += (12 + (int64_t)sizeof(HubItem)) * g->nhub_items + (8 + 8 * kHubTile) * g->nhub_tiles +
+                4 * ntile_idx;
         if (g->reorder) need += 8 * n + (ooc ? 0 : 4 * m + 8 * (n + 1));  // perm arrays + transient original CSR
         else need += 4 * n;
         size_t fr = 0, tot = 0;
@@ -428,6 +432,8 @@ static int upload_impl(galp_graph *g, const galp_csr_view *c) {
         GALP_TRY(dev_alloc(g, &g->d_item_part, g->nhub_items));
         GALP_HIP(ctx, up(g->d_item_hub, item_hub.data(), 4 * item_hub.size()));
         GALP_HIP(ctx, up(g->d_item_part, item_part.data(), 4 * item_part.size()));
+        GALP_TRY(dev_alloc(g, &g->d_item_meta, g->nhub_items));
+        GALP_HIP(ctx, up(g->d_item_meta, item_meta.data(), sizeof(HubItem) * item_meta.size()));
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_hub_best, 2 * g->nhubs, 0u));
         GALP_TRY(dev_fill_u32(g, (uint32_t *)g->d_merge_big, g->nhub_items, 0u));
     }

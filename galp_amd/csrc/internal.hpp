@@ -60,6 +60,15 @@ constexpr int kHubMaxParts = 4096;
 constexpr int kHubPartSort = 512;
 constexpr int kHubMergeCapBits = 12;  // LDS table of one partition (4096 slots)
 
+// A merge item (hub, label partition) with what its merge reads first, in
+// one 32-byte record: the hub's tiles are tile0 .. tile0 + ntiles - 1, each
+// holding nparts + 1 partition offsets, consecutive from tile_idx[idx0].
+struct alignas(16) HubItem {
+    int32_t hub, part, tile0, ntiles;
+    int32_t nparts, pad;
+    int64_t idx0;
+};
+
 // Parts of the neighbour-id space: (u >> kPartLineBits) mod kNbrParts, the
 // 128-byte label line of u modulo kNbrParts.  Rows stored grouped by part
 // and tiles listed part-major let the device gather from one part's lines
@@ -120,6 +129,7 @@ constexpr int kCtrCandVerts = kCtrOverflow + 1;
 constexpr int kCtrPlainHubs = kCtrOverflow + 2;           // hubs left to the plain hub path
 constexpr int kCtrCandMt = kCtrOverflow + 3;              // multi-tile vertices routed to candidates
 constexpr int kCtrHd = kCtrOverflow + 4;                  // NUM_BUCKETS sizes of the high-distinct lists
+                                                          // (B_HUB's: hubs of the high-distinct hub launch)
 constexpr int kCtrHdAll = kCtrHd + NUM_BUCKETS;           // big vertices predicted high-distinct (any path)
 constexpr int kNumCtrs = kCtrHdAll + 1;
 // A vertex whose predicted distinct-label count exceeds its degree / kHdDiv
@@ -278,6 +288,7 @@ struct galp_graph {
     uint32_t *d_out_key = nullptr, *d_out_ent = nullptr;   // kHubTile per tile: label, hub_entry
     int32_t *d_item_hub = nullptr;          // hub of each merge item
     int32_t *d_item_part = nullptr;         // partition of each merge item
+    galp::HubItem *d_item_meta = nullptr;        // the same, with the hub's tile range and offsets
     unsigned long long *d_hub_best = nullptr;  // max (count << 32 | last) per hub
     // per merge item, set by the wave merge in every iteration: 1 when the
     // item has more entries (or distinct labels) than one wave merges, and

@@ -177,6 +177,9 @@ k_dispatch(const E *__restrict__ off, const int32_t *__restrict__ vlist, int64_t
                 hub_sel[mt] = cand ? 0 : 1;
                 hub_hd[mt] = (!cand && hd) ? 1 : 0;
                 if (!cand) atomicAdd(&scount[kSlotHubs], 1ull);
+                // the hub class has no list of its own: its high-distinct
+                // slot counts the hubs of the hub path's high-distinct launch
+                if (!cand && hd) atomicAdd(&scount[NUM_BUCKETS + B_HUB], 1ull);
             }
             code = cand ? (uint8_t)kCodeCandMt : cls < B_HUB ? (uint8_t)list : (uint8_t)kCodeHub;
         } else {

@@ -486,7 +486,7 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
            uint32_t *__restrict__ out_key, uint32_t *__restrict__ out_ent,
            int32_t *__restrict__ tile_idx, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
            const int32_t *__restrict__ hub_sel, const int64_t *__restrict__ nsel,
-           const int32_t *__restrict__ hub_hd) {
+           const int32_t *__restrict__ hub_hd, const int64_t *__restrict__ nhd) {
     constexpr int CAP = 1 << kHubTileCapBits;
     constexpr int EPT = kHubTile / T;
     // keys | counts | positions; the partition histogram (P + 1 entries)
@@ -494,7 +494,11 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
     static_assert(kHubMaxParts + 1 <= 2 * CAP, "partition histogram fits in the table");
     __shared__ uint32_t tab[3 * CAP];
     __shared__ int sscan[T / kWave];
-    if (*stop || (nsel && *nsel == 0)) return;   // no hub on this path
+    if (*stop) return;
+    if (nsel) {   // learned dispatch: nsel hubs on this path, nhd of them high-distinct
+        const int64_t ns = *nsel, nh = nhd ? *nhd : 0;
+        if ((hub_hd ? (HD ? nh : ns - nh) : ns) == 0) return;   // none for this launch
+    }
     for (int s = threadIdx.x; s < CAP; s += T) {
         tab[s] = kEmpty;
         tab[CAP + s] = 0;
@@ -515,35 +519,50 @@ k_hub_tile(const E *__restrict__ off, const int32_t *__restrict__ nbr, const int
 // counts and a wave max of its last positions), keeping up to 64 distinct
 // labels in lanes (lane j: label, count, last position of candidate j).
 // Items with more entries, or more than 64 distinct labels, are flagged for
-// the block merge (k_hub_merge); waves need no barrier.
+// the block merge (k_hub_merge), every other item's flag is cleared (hubs
+// outside the launch's selection included); waves need no barrier.  Each
+// wave loads its next item's record while it merges the current one.
 constexpr int kHmSmall = 1024;
 constexpr int kHmSmallT = 256;
 
 __global__ void __launch_bounds__(kHmSmallT)
-k_hub_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
-                  int64_t nitems, const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
-                  const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
-                  const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
-                  unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ hub_sel,
-                  uint32_t *__restrict__ hub_distinct, const int64_t *__restrict__ nsel, int32_t *__restrict__ big,
-                  const int32_t *__restrict__ stop) {
+k_hub_merge_small(const HubItem *__restrict__ items, int64_t ibase, int64_t nitems,
+                  const int32_t *__restrict__ tile_idx, const uint32_t *__restrict__ out_key,
+                  const uint32_t *__restrict__ out_ent, unsigned long long *__restrict__ hub_best,
+                  const int32_t *__restrict__ hub_sel, uint32_t *__restrict__ hub_distinct,
+                  const int64_t *__restrict__ nsel, int32_t *__restrict__ big, const int32_t *__restrict__ stop) {
     constexpr int NW = kHmSmallT / kWave;
     __shared__ int32_t slo[NW][kWave], sstart[NW][kWave];
     if (*stop || (nsel && *nsel == 0)) return;
     const int lane = lane_id(), w = (int)(threadIdx.x / kWave);
-    const int64_t wid = (int64_t)blockIdx.x * NW + w;
-    for (int64_t it = ibase + wid; it < ibase + nitems; it += (int64_t)gridDim.x * NW) {
-        const int h = item_hub[it];
-        if (hub_sel && !hub_sel[h]) continue;   // uniform per wave
-        const int p = item_part[it];
-        const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
-        int total = 0;
+    const int64_t iend = ibase + nitems, stride = (int64_t)gridDim.x * NW;
+    int64_t it = ibase + (int64_t)blockIdx.x * NW + w;
+    HubItem nx{};   // the wave's next item, loaded one item ahead
+    if (it < iend) nx = items[it];
+    for (; it < iend; it += stride) {
+        const HubItem m = nx;
+        if (it + stride < iend) nx = items[it + stride];
+        const int h = m.hub;
+        if (hub_sel && !hub_sel[h]) {   // uniform per wave; the block merge skips it too
+            if (lane == 0) big[it] = 0;
+            continue;
+        }
+        const int p = m.part;
+        const int64_t t0 = m.tile0, nt = m.ntiles, pstride = (int64_t)m.nparts + 1;
+        // slice bounds of partition p in each tile, one tile per lane; those
+        // of the first 64 tiles stay in registers for the pass below
+        int lo0 = 0, len0 = 0, total = 0;
         for (int64_t c0 = t0; c0 < t0 + nt; c0 += kWave) {
             const int64_t t = c0 + lane;
-            int len = 0;
+            int lo = 0, len = 0;
             if (t < t0 + nt) {
-                const int64_t ib = tile_idx_off[t];
-                len = tile_idx[ib + p + 1] - tile_idx[ib + p];
+                const int64_t ib = m.idx0 + (t - t0) * pstride + p;
+                lo = tile_idx[ib];
+                len = tile_idx[ib + 1] - lo;
+            }
+            if (c0 == t0) {
+                lo0 = lo;
+                len0 = len;
             }
             total += (int)wave_sum_u32((uint32_t)len);
         }
@@ -557,11 +576,15 @@ k_hub_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restric
         int ncand = 0;      // uniform
         for (int64_t c0 = t0; c0 < t0 + nt && !over; c0 += kWave) {
             const int64_t t = c0 + lane;
-            int lo = 0, len = 0;
-            if (t < t0 + nt) {
-                const int64_t ib = tile_idx_off[t];
-                lo = tile_idx[ib + p];
-                len = tile_idx[ib + p + 1] - lo;
+            int lo = lo0, len = len0;
+            if (c0 != t0) {
+                lo = 0;
+                len = 0;
+                if (t < t0 + nt) {
+                    const int64_t ib = m.idx0 + (t - t0) * pstride + p;
+                    lo = tile_idx[ib];
+                    len = tile_idx[ib + 1] - lo;
+                }
             }
             int st = len;   // exclusive scan of the slice lengths
 #pragma unroll
@@ -633,16 +656,21 @@ k_hub_merge_small(const int32_t *__restrict__ item_hub, const int32_t *__restric
 
 // LIST: keep a list of the slots created (argmax and reset visit only them);
 // without it they scan the whole table, which frees 16 KB of LDS per block.
+// A block takes its flagged items one after another, so each item's global
+// loads are issued ahead of their use: the next item's record when an item
+// starts, the next item's first slice bounds once this item's entry loads
+// are in flight, and the entries EPT per thread at a time.
+// Without the list, three 512-thread blocks fit a CU's LDS; their 24 waves
+// need at most 80 VGPRs each (six waves per SIMD).
 template <int T, bool LIST>
-__global__ void __launch_bounds__(T)
-k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ item_part, int64_t ibase,
-            int64_t nitems, const int32_t *__restrict__ big,
-            const int32_t *__restrict__ hub_tile0, const int32_t *__restrict__ hub_ntiles,
-            const int64_t *__restrict__ tile_idx_off, const int32_t *__restrict__ tile_idx,
-            const uint32_t *__restrict__ out_key, const uint32_t *__restrict__ out_ent,
-            unsigned long long *__restrict__ hub_best, const int32_t *__restrict__ stop, int32_t *__restrict__ err,
-            const int32_t *__restrict__ hub_sel, uint32_t *__restrict__ hub_distinct, const int64_t *__restrict__ nsel) {
+__global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(!LIST && T == 512 ? 6 : 1)))
+k_hub_merge(const HubItem *__restrict__ items, int64_t ibase, int64_t nitems, const int32_t *__restrict__ big,
+            const int32_t *__restrict__ tile_idx, const uint32_t *__restrict__ out_key,
+            const uint32_t *__restrict__ out_ent, unsigned long long *__restrict__ hub_best,
+            const int32_t *__restrict__ stop, int32_t *__restrict__ err, uint32_t *__restrict__ hub_distinct,
+            const int64_t *__restrict__ nsel) {
     constexpr int CAP = 1 << kHubMergeCapBits;
+    constexpr int EPT = 4;   // entries per thread and pass
     __shared__ uint32_t skey[CAP];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t slast[CAP];
@@ -662,11 +690,11 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
     }
     if (threadIdx.x == 0) nlist = 0;
     if (threadIdx.x == 0) sfull = 0;
-    // the flagged items of each chunk of T of the block's items, gathered
-    // into an LDS bit mask (one flag load per thread instead of a dependent
-    // load per item)
     // items go round-robin over the blocks (item ibase + blockIdx.x + j *
-    // gridDim.x is the block's j-th), T of them per chunk
+    // gridDim.x is the block's j-th), T of them per chunk; the flags of a
+    // chunk are gathered into an LDS bit mask (one flag load per thread
+    // instead of a dependent load per item).  The wave merge has cleared the
+    // flag of every item outside this launch's hub selection.
     const int64_t iend = ibase + nitems;
     for (int64_t j0 = 0; ibase + blockIdx.x + j0 * gridDim.x < iend; j0 += T) {
     __syncthreads();   // the previous chunk's reads of sbig are done
@@ -677,18 +705,29 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         if (ij < iend && big[ij]) atomicOr(&sbig[threadIdx.x / 32], 1u << (threadIdx.x % 32));
     }
     __syncthreads();
-    for (int q = 0; q < T; ++q) {
-        const uint32_t word = sbig[q / 32] >> (q % 32);   // uniform
-        if (word == 0u) {   // no flagged item left in this word
-            q |= 31;
-            continue;
+    // the first flagged item of the chunk at or after q (T: none); uniform,
+    // so item records load through the scalar cache into SGPRs
+    auto next_q = [&](int q) {
+        while (q < T) {
+            const uint32_t word = sbig[q / 32] >> (q % 32);
+            if (word) return __builtin_amdgcn_readfirstlane(q + __ffs((int)word) - 1);
+            q = (q | 31) + 1;
         }
-        q += __ffs((int)word) - 1;
-        const int64_t it = ibase + blockIdx.x + (j0 + q) * gridDim.x;
-        const int h = item_hub[it];
-        if (hub_sel && !hub_sel[h]) continue;   // uniform per block
-        const int p = item_part[it];
-        const int64_t t0 = hub_tile0[h], nt = hub_ntiles[h];
+        return T;
+    };
+    int q = next_q(0);
+    HubItem nx{};
+    if (q < T) nx = items[ibase + blockIdx.x + (j0 + q) * gridDim.x];
+    int plo = 0, phi = 0;   // bounds of the next item's slice in tile tile0 + threadIdx.x
+    bool pre = false;       // plo / phi hold them
+    while (q < T) {
+        const HubItem m = nx;
+        const int qn = next_q(q + 1);
+        if (qn < T) nx = items[ibase + blockIdx.x + (j0 + qn) * gridDim.x];
+        bool have = pre;   // this item's first slice bounds are in plo / phi
+        pre = false;
+        const int h = m.hub, p = m.part;
+        const int64_t t0 = m.tile0, nt = m.ntiles, pstride = (int64_t)m.nparts + 1;
         // A partition with more distinct labels than the table holds (an
         // adversarial label set, or a hub whose partitions were capped at
         // kHubMaxParts) is merged again in R rounds, round r taking the
@@ -706,10 +745,14 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
             __syncthreads();
             const int64_t t = c0 + threadIdx.x;
             int lo = 0, len = 0;
-            if (t < t0 + nt) {
-                const int64_t ib = tile_idx_off[t];
-                lo = tile_idx[ib + p];
-                len = tile_idx[ib + p + 1] - lo;
+            if (have) {   // uniform: the first chunk of the item's first round
+                lo = plo;
+                len = phi - plo;
+                have = false;
+            } else if (t < t0 + nt) {
+                const int64_t ib = m.idx0 + (t - t0) * pstride + p;
+                lo = tile_idx[ib];
+                len = tile_idx[ib + 1] - lo;
             }
             const int start = block_exclusive_scan<T / kWave>(len, sscan_w);   // barriers inside
             slo[threadIdx.x] = lo;
@@ -718,38 +761,60 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
             __syncthreads();
             const int total = stotal;
             const int nc = (int)min((int64_t)T, t0 + nt - c0);
-            for (int i0 = 0; i0 < total; i0 += T) {   // uniform trip count: ballots below
-                const int i = i0 + (int)threadIdx.x;
-                bool created = false;
-                uint32_t sl = 0;
-                if (i < total) {
-                    int a = 0, b = nc;   // last tile with start <= i
-                    while (b - a > 1) {
-                        const int mid = (a + b) >> 1;
-                        if (shi[mid] <= i) a = mid;
-                        else b = mid;
+            for (int i0 = 0; i0 < total; i0 += T * EPT) {   // uniform trip count: ballots below
+                uint32_t key[EPT], ent[EPT], tb[EPT];   // key kEmpty: no entry
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    const int i = i0 + k * T + (int)threadIdx.x;
+                    key[k] = kEmpty;
+                    ent[k] = 0u;
+                    tb[k] = 0u;
+                    if (i < total) {
+                        int a = 0, b = nc;   // last tile with start <= i
+                        while (b - a > 1) {
+                            const int mid = (a + b) >> 1;
+                            if (shi[mid] <= i) a = mid;
+                            else b = mid;
+                        }
+                        const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - shi[a]);
+                        key[k] = out_key[e];
+                        ent[k] = out_ent[e];
+                        tb[k] = (uint32_t)((c0 + a - t0) * (int64_t)kHubTile);
                     }
-                    const int64_t e = (c0 + a) * (int64_t)kHubTile + slo[a] + (i - shi[a]);
-                    const uint32_t key = out_key[e];
-                    if (R == 1 || sub_round(key, R) == r) {
-                        const uint32_t w = out_ent[e];
-                        sl = lds_try_insert<kHubMergeCapBits>(skey, key, &created);
+                }
+                if (!pre && qn < T) {   // uniform: once per item, behind its first entry loads
+                    plo = 0;
+                    phi = 0;
+                    if ((int64_t)threadIdx.x < (int64_t)nx.ntiles) {
+                        const int64_t ib =
+                            nx.idx0 + (int64_t)threadIdx.x * ((int64_t)nx.nparts + 1) + nx.part;
+                        plo = tile_idx[ib];
+                        phi = tile_idx[ib + 1];
+                    }
+                    pre = true;
+                }
+#pragma unroll
+                for (int k = 0; k < EPT; ++k) {
+                    bool created = false;
+                    uint32_t sl = 0;
+                    if (key[k] != kEmpty && (R == 1 || sub_round(key[k], R) == r)) {
+                        sl = lds_try_insert<kHubMergeCapBits>(skey, key[k], &created);
                         if (sl == kTableFull) {
                             sfull = 1;
                         } else {
-                            atomicAdd(&scnt[sl], hub_entry_count(w));
-                            atomicMax(&slast[sl], (uint32_t)((c0 + a - t0) * (int64_t)kHubTile) + hub_entry_pos(w));
+                            atomicAdd(&scnt[sl], hub_entry_count(ent[k]));
+                            atomicMax(&slast[sl], tb[k] + hub_entry_pos(ent[k]));
                         }
                     }
-                }
-                // one list append per wave
-                const unsigned long long ball = LIST ? __ballot(created) : 0ull;
-                if (ball) {
-                    const int leader = __ffsll((long long)ball) - 1;
-                    uint32_t base = 0;
-                    if (lane_id() == leader) base = atomicAdd(&nlist, (uint32_t)__popcll(ball));
-                    base = (uint32_t)__shfl((int)base, leader);
-                    if (created) slist[base + __popcll(ball & ((1ull << lane_id()) - 1ull))] = sl;
+                    // one list append per wave
+                    const unsigned long long ball = LIST ? __ballot(created) : 0ull;
+                    if (ball) {
+                        const int leader = __ffsll((long long)ball) - 1;
+                        uint32_t base = 0;
+                        if (lane_id() == leader) base = atomicAdd(&nlist, (uint32_t)__popcll(ball));
+                        base = (uint32_t)__shfl((int)base, leader);
+                        if (created) slist[base + __popcll(ball & ((1ull << lane_id()) - 1ull))] = sl;
+                    }
                 }
             }
         }
@@ -806,6 +871,7 @@ k_hub_merge(const int32_t *__restrict__ item_hub, const int32_t *__restrict__ it
         }
         if (threadIdx.x == 0 && item_best) atomicMax(&hub_best[h], item_best);
         if (threadIdx.x == 0 && hub_distinct) atomicAdd(&hub_distinct[h], item_nl);
+        q = qn;
     }
     }
 }
@@ -930,28 +996,27 @@ static void launch_hubs(galp_graph *g, const IterView &w, const int32_t *prev, i
     const int32_t *sel = learned_now(g) ? g->learn->d_hub_sel : nullptr;
     uint32_t *hdist = distinct_out(g) ? g->learn->d_hub_distinct : nullptr;
     const int64_t *nsel = sel ? g->learn->d_plist_n + kCtrPlainHubs : nullptr;
+    const int64_t *nhd = sel ? g->learn->d_plist_n + kCtrHd + B_HUB : nullptr;
     PendingLaunch pl;
     prof_begin(g, "hub_tile", 8.0 * hub_edges, &pl);
     // tiles [t0, t1): the pipeline starts at t0 + blockIdx.x, stops at t1
     k_hub_tile<TT, E, false><<<grid_for(g, nt, 3), TT, 0, g->stream>>>(
         (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
         g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
-        g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr);
+        g->d_err, sel, nsel, sel ? g->learn->d_hub_hd : nullptr, nhd);
     if (sel)   // learned dispatch: the hubs predicted high-distinct
         k_hub_tile<TT, E, true><<<grid_for(g, nt, 3), TT, 0, g->stream>>>(
             (const E *)w.off, w.nbr, prev, g->d_tile_hub, g->d_hub_tile0, g->d_hub_vid, g->d_hub_nparts,
             g->d_tile_idx_off, w.t0, w.t1, g->d_out_key, g->d_out_ent, g->d_tile_idx, g->d_stop,
-            g->d_err, sel, nsel, g->learn->d_hub_hd);
+            g->d_err, sel, nsel, g->learn->d_hub_hd, nhd);
     prof_end(g, &pl);
     prof_begin(g, "hub_merge", 0.0, &pl);
     k_hub_merge_small<<<grid_for(g, (ni + kHmSmallT / kWave - 1) / (kHmSmallT / kWave), 8), kHmSmallT, 0,
-                        g->stream>>>(g->d_item_hub, g->d_item_part, w.i0, ni, g->d_hub_tile0, g->d_hub_ntiles,
-                                     g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best,
-                                     sel, hdist, nsel, g->d_merge_big, g->d_stop);
+                        g->stream>>>(g->d_item_meta, w.i0, ni, g->d_tile_idx, g->d_out_key, g->d_out_ent,
+                                     g->d_hub_best, sel, hdist, nsel, g->d_merge_big, g->d_stop);
     k_hub_merge<TM, LIST><<<grid_for(g, ni, TM == 256 || !LIST ? 3 : 2), TM, 0, g->stream>>>(
-        g->d_item_hub, g->d_item_part, w.i0, ni, g->d_merge_big, g->d_hub_tile0, g->d_hub_ntiles,
-        g->d_tile_idx_off, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best, g->d_stop, g->d_err, sel,
-        hdist, nsel);
+        g->d_item_meta, w.i0, ni, g->d_merge_big, g->d_tile_idx, g->d_out_key, g->d_out_ent, g->d_hub_best,
+        g->d_stop, g->d_err, hdist, nsel);
     prof_end(g, &pl);
     prof_begin(g, "hub_final", (double)sizeof(E) * 2.0 * (double)nh + 16.0 * (double)nh, &pl);
     k_hub_final<E><<<grid_for(g, (nh + 255) / 256, 4), 256, 0, g->stream>>>(
