@@ -12,9 +12,10 @@
 //     last positions come from lane exchanges, no table at all;
 //   - 65..4096: one wave (or one workgroup of up to 512 threads) per vertex
 //     with an open-addressing table in LDS at load factor <= 1/2;
-//   - hubs (> 4096): ceil(deg/2048) blocks per hub, each counting one hash
-//     partition of the labels in LDS over the whole row; a 64-bit atomicMax
-//     of (count, last) per block picks the winner, no global table at all.
+//   - hubs (> 4096): tiles of 2048 edges counted in LDS, each writing its
+//     distinct labels grouped by hash partition; a merge per (hub,
+//     partition) sums them in LDS, and a 64-bit atomicMax of (count, last)
+//     per merge picks the winner: no device-scope table at all.
 // Table-based kernels run a software pipeline over their vertices: the
 // labels of the next vertex and the neighbour ids of the one after are in
 // flight while the current vertex is counted, so the dependent
