@@ -501,7 +501,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{workload}, {args.iters} synchronous LP iterations per step, " +
                                (f"out-of-core (batches < 2^{args.buffer_pow} edges)" if "_ooc_" in head else "in-core"),
-                   "n": n, "m": m, "method": "hash (method 1)" if head != "sort" else "sort (method 0)",
+                   "n": n, "m": m, "method": "sort (method 0)" if head.startswith("sort") else "hash (method 1)",
                    "dispatch": "learned" if head.startswith("hash_learned") else "plain",
                    "autotuned": head.endswith("_tuned"),
                    "iterations_per_step": iters // args.steps},
