@@ -250,7 +250,10 @@ GALP_API int galp_graph_features(galp_graph *g, float *deg, float *mean, float *
 /* Counters of the last iteration routed by the network: [0..12) class list
  * sizes, [12..16) candidate tiles per length class (<= 128, 256, 512, 1024
  * edges), [16] candidate overflows, [17] vertices routed to candidates,
- * [18] hubs left to the plain hub path. */
+ * [18] hubs left to the plain hub path, [19] multi-tile rows routed to
+ * candidates, [20..32) sizes of the classes' high-distinct lists ([31], the
+ * hub class: hubs of the high-distinct hub launch), [32] big vertices
+ * predicted high-distinct on any path. */
 GALP_API int galp_graph_dispatch_stats(galp_graph *g, int64_t *out, int32_t max, int32_t *count);
 
 /* ---- autotuner -------------------------------------------------------------
